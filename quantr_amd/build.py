@@ -1,0 +1,73 @@
+"""Build recipe for the gfx950 engine (libquantr_hip.so) and the C++ API module.
+
+Everything is compiled in-tree into quantr_amd/lib/ so the built libraries travel with the
+repository snapshot. hipcc cross-compiles for gfx950 without a GPU present.
+"""
+import concurrent.futures
+import os
+import shutil
+import subprocess
+import sys
+import sysconfig
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "lib")
+INCLUDE = os.path.join(ROOT, "include")
+BUILD = os.path.join(HERE, "_build")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# Packed FP32 instructions are disabled: hipcc must not form v_pk_*_f32 on its own.
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC,
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
+]
+SOURCES = ["kernels.hip", "lower.cpp", "planner.cpp", "capi.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l and l.strip())
+    if r.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s" % (r.returncode, " ".join(cmd), out))
+    return out
+
+
+def build_engine(verbose=False):
+    os.makedirs(BUILD, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in SOURCES:
+        obj = os.path.join(BUILD, src + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        for out in ex.map(_run, cmds):
+            if verbose and out:
+                print(out)
+    so = os.path.join(LIB, "libquantr_hip.so")
+    tmp = so + ".tmp"
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
+         ["-Wl,-soname,libquantr_hip.so"])
+    os.replace(tmp, so)
+    return so
+
+
+def build_oracle():
+    odir = os.path.join(ROOT, "oracle")
+    _run(["make", "-C", odir, "-s", "_build/liboracle.so"])
+    return os.path.join(odir, "_build", "liboracle.so")
+
+
+def build_all(verbose=False):
+    so = build_engine(verbose)
+    orc = build_oracle()
+    return so, orc
+
+
+if __name__ == "__main__":
+    print(build_all(verbose=True))

@@ -1,0 +1,117 @@
+// engine.hpp — internal declarations of the gfx950 state-vector engine.
+#pragma once
+
+#include <cstdint>
+#include <complex>
+#include <string>
+#include <vector>
+
+#include "quantr_hip.h"
+#include "kernels.hpp"
+
+namespace qr {
+
+using cd = std::complex<double>;
+
+// ---- errors -----------------------------------------------------------------------------
+void set_error(const std::string& msg);
+void clear_error();
+
+struct Error {
+  qr_status status;
+  std::string msg;
+};
+
+// ---- gate lowering ------------------------------------------------------------------------
+
+// A gate exactly as quantr applies it: the matrix over its acting wires
+// [controls..., position] (MSB first), column j = image of local basis state j, with the
+// custom-gate overwrite rule folded in (simulation.rs:108-133).
+struct ExactGate {
+  int kind = 0;
+  int K = 0;                  // acting wires
+  std::vector<int> bits;      // global bit positions (n-1-wire), local MSB first
+  std::vector<cd> M;          // M[r*dim + j]
+  std::vector<uint8_t> some;  // input j contributes (always 1 for standard gates)
+  std::vector<uint8_t> keep;  // output r is overwritten by its input (custom None rows)
+  bool custom = false;
+  double arg = 0.0;
+};
+
+// Fast micro-op kinds used by the fused and unfused-fast paths.
+enum FastKind : int {
+  FK_DENSE1 = 0,  // 2x2 on one target bit (+ controls)
+  FK_DIAG1 = 1,   // diag(d0, d1) on one bit (+ controls)
+  FK_DENSEK = 2,  // 2^k x 2^k on k target bits (+ controls)
+  FK_DIAGK = 3,   // 2^k phases on k bits (+ controls)
+};
+
+// Specialisation hints of a 2x2 (kernels pick a cheaper form when exact).
+enum U1Form : int {
+  U1_GENERAL = 0,
+  U1_PERM = 1,   // anti-diagonal [[0,b],[c,0]] (X, Y, X90, ...)
+  U1_REAL = 2,   // real entries (H, Ry)
+};
+
+struct FastOp {
+  int kind = FK_DENSE1;
+  int k = 1;
+  int bits[8] = {0};    // target bits, local MSB first
+  uint64_t cmask = 0;   // control bits that must be 1
+  std::vector<cd> m;    // DENSE1: 4 row-major; DIAG1: 2; DENSEK: dim*dim; DIAGK: dim
+  int form = U1_GENERAL;
+  // Differentiation: index of the angle parameter (-1 if constant) and its generator
+  // (1 = X, 2 = Y, 3 = Z, 4 = identity / global phase): U = exp(-i theta G / 2), or
+  // exp(+i theta / 2) I for the global phase.
+  int param = -1;
+  int gen = 0;
+  int gate_index = -1;  // position in the program (for dependency order)
+  double f_touch = 1.0; // fraction of amplitudes this op must read and write
+  uint64_t target_mask() const;
+  uint64_t read_mask() const;  // controls and diagonal bits
+};
+
+// Validates and lowers a program. Throws Error.
+void validate_program(uint32_t n, const qr_program* p);
+std::vector<ExactGate> lower_exact(uint32_t n, const qr_program* p);
+std::vector<FastOp> lower_fast(uint32_t n, const std::vector<ExactGate>& gates, bool want_params);
+// Human-readable form of an op (Rust Debug style, e.g. "CNot(2)").
+std::string gate_debug(const qr_op& op, const uint32_t* controls);
+
+// Adjoint of a fast op (for the reverse sweep).
+FastOp adjoint(const FastOp& op);
+
+// ---- planning -------------------------------------------------------------------------------
+
+struct TileOpHost {
+  uint32_t code;
+  uint32_t a;
+  uint64_t cmask;
+  uint32_t coef;   // index into the coefficient array (complex units)
+  uint32_t b;
+  uint32_t c;
+  uint32_t d;
+};
+
+struct Stage {
+  bool tile = true;            // fused tile sweep, or a standalone op
+  int k = 0;                   // tile bits
+  std::vector<int> tbits;      // tile bits ascending (global positions)
+  std::vector<int> maps;       // flat: maps[i*k + s] = tile position held by slot s
+  std::vector<TileOpHost> ops;
+  std::vector<cd> coefs;
+  int standalone = -1;         // index into the op list for standalone stages
+  double bytes = 0.0;          // algorithmic bytes of this sweep (2 * 2^n * B * f_touch)
+  int ngates = 0;
+};
+
+struct PlanParams {
+  int tile_bits = 12;
+  int reg_bits = 4;
+  int low_bits = 4;
+};
+
+std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
+                              int bytes_per_amp, bool dual);
+
+}  // namespace qr

@@ -1,0 +1,999 @@
+// kernels.hip — gfx950 kernels of the state-vector engine (wave64, CDNA4).
+//
+// State layout: 2^n amplitudes, interleaved (re, im), index i = sum_w bit_w * 2^(n-1-w)
+// (wire 0 = most significant bit, product_states.rs:191-215). complex128 amplitudes are
+// loaded as one 16-byte dwordx4, complex64 as one 8-byte dwordx2.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace qr {
+namespace dev {
+
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using T = double2;
+};
+template <>
+struct CT<float> {
+  using T = float2;
+};
+
+// Inserts a zero bit at each position of `sorted_bits` (ascending) into x.
+__device__ __forceinline__ uint64_t insert_zero_bits(uint64_t x, const uint8_t* sorted_bits, int nb) {
+  for (int q = 0; q < nb; ++q) {
+    const uint64_t b = sorted_bits[q];
+    const uint64_t lo = x & ((1ull << b) - 1);
+    x = ((x >> b) << (b + 1)) | lo;
+  }
+  return x;
+}
+
+// =====================================================================================
+// |0...0> (SuperPosition::new_unchecked, super_positions_unchecked.rs:39-46)
+// =====================================================================================
+template <typename R>
+__global__ void k_init_basis(typename CT<R>::T* __restrict__ psi, uint64_t dim, uint64_t one_at) {
+  using C = typename CT<R>::T;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    C v;
+    v.x = (i == one_at) ? R(1) : R(0);
+    v.y = R(0);
+    psi[i] = v;
+  }
+}
+
+// =====================================================================================
+// Exact per-gate sweep: new[r] = sum over inputs j in ascending global index order of
+// M[r][j] * a[j], the first product taken as is and the rest added, complex products in
+// the textbook form with no fused multiply-add (Circuit::apply_gate, simulation.rs:64-180).
+// Inputs whose custom image is None keep their amplitude (simulation.rs:120-133).
+// One thread per group of 2^K amplitudes that differ only in the acting bits.
+// =====================================================================================
+template <typename R, int KR>
+__global__ void k_exact(typename CT<R>::T* __restrict__ psi, uint64_t ngroups, ExactArgs a,
+                        const double2* __restrict__ M) {
+#pragma clang fp contract(off)
+  using C = typename CT<R>::T;
+  const int K = a.K;
+  const int dim = 1 << K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t base = insert_zero_bits(g, a.sorted_bits, K);
+    if (KR > 0) {
+      C in[1 << (KR > 0 ? KR : 1)];
+#pragma unroll
+      for (int j = 0; j < (1 << KR); ++j)
+        if (j < dim) in[j] = psi[base + a.off[j]];
+      C out[1 << (KR > 0 ? KR : 1)];
+#pragma unroll
+      for (int r = 0; r < (1 << KR); ++r) {
+        if (r >= dim) continue;
+        if ((a.keep >> r) & 1) {
+          out[r] = in[r];
+          continue;
+        }
+        R acc_re = R(0), acc_im = R(0);
+        bool first = true;
+#pragma unroll
+        for (int oi = 0; oi < (1 << KR); ++oi) {
+          if (oi >= dim) continue;
+          const int j = a.order[oi];
+          if (!((a.some >> j) & 1)) continue;
+          const double2 m = M[r * dim + j];
+          const R mr = (R)m.x, mi = (R)m.y;
+          C v;
+#pragma unroll
+          for (int jj = 0; jj < (1 << KR); ++jj)
+            if (jj == j) v = in[jj];
+          const R pr = mr * v.x - mi * v.y;
+          const R pi = mr * v.y + mi * v.x;
+          if (first) {
+            acc_re = pr;
+            acc_im = pi;
+            first = false;
+          } else {
+            acc_re = acc_re + pr;
+            acc_im = acc_im + pi;
+          }
+        }
+        out[r].x = acc_re;
+        out[r].y = acc_im;
+      }
+#pragma unroll
+      for (int j = 0; j < (1 << KR); ++j)
+        if (j < dim) psi[base + a.off[j]] = out[j];
+    }
+  }
+}
+
+// Generic exact sweep for wide custom gates (K > 4): inputs re-read from memory.
+template <typename R>
+__global__ void k_exact_wide(typename CT<R>::T* __restrict__ psi, typename CT<R>::T* __restrict__ tmp,
+                             uint64_t ngroups, ExactWideArgs a, const double2* __restrict__ M,
+                             const uint64_t* __restrict__ off, const uint32_t* __restrict__ order,
+                             const uint8_t* __restrict__ flags) {
+#pragma clang fp contract(off)
+  using C = typename CT<R>::T;
+  const int dim = 1 << a.K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  // flags[j] bit0 = some, bit1 = keep
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t base = insert_zero_bits(g, a.sorted_bits, a.K);
+    C* out = tmp + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * (uint64_t)dim;
+    for (int r = 0; r < dim; ++r) {
+      if (flags[r] & 2) {
+        out[r] = psi[base + off[r]];
+        continue;
+      }
+      R acc_re = R(0), acc_im = R(0);
+      bool first = true;
+      for (int oi = 0; oi < dim; ++oi) {
+        const int j = (int)order[oi];
+        if (!(flags[j] & 1)) continue;
+        const double2 m = M[(uint64_t)r * dim + j];
+        const C v = psi[base + off[j]];
+        const R mr = (R)m.x, mi = (R)m.y;
+        const R pr = mr * v.x - mi * v.y;
+        const R pi = mr * v.y + mi * v.x;
+        if (first) {
+          acc_re = pr;
+          acc_im = pi;
+          first = false;
+        } else {
+          acc_re = acc_re + pr;
+          acc_im = acc_im + pi;
+        }
+      }
+      C o;
+      o.x = acc_re;
+      o.y = acc_im;
+      out[r] = o;
+    }
+    for (int r = 0; r < dim; ++r) psi[base + off[r]] = out[r];
+  }
+}
+
+// =====================================================================================
+// Fast standalone sweeps (fused multiply-add allowed)
+// =====================================================================================
+
+template <typename R>
+__device__ __forceinline__ void cmul_acc(R& xr, R& xi, R ur, R ui, R vr, R vi) {
+  xr = fma(ur, vr, fma(-ui, vi, xr));
+  xi = fma(ur, vi, fma(ui, vr, xi));
+}
+
+// 2x2 on target bit t for every pair whose control bits are all 1.
+template <typename R>
+__global__ void k_dense1(typename CT<R>::T* __restrict__ psi, uint64_t npairs, int t, uint64_t cmask,
+                         double2 u00, double2 u01, double2 u10, double2 u11) {
+  using C = typename CT<R>::T;
+  const uint64_t tb = 1ull << t;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  const R a_r = u00.x, a_i = u00.y, b_r = u01.x, b_i = u01.y, c_r = u10.x, c_i = u10.y, d_r = u11.x,
+          d_i = u11.y;
+  for (uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p < npairs; p += stride) {
+    const uint64_t i0 = ((p >> t) << (t + 1)) | (p & (tb - 1));
+    if ((i0 & cmask) != cmask) continue;
+    const uint64_t i1 = i0 | tb;
+    const C x = psi[i0], y = psi[i1];
+    R o0r = 0, o0i = 0, o1r = 0, o1i = 0;
+    cmul_acc(o0r, o0i, a_r, a_i, (R)x.x, (R)x.y);
+    cmul_acc(o0r, o0i, b_r, b_i, (R)y.x, (R)y.y);
+    cmul_acc(o1r, o1i, c_r, c_i, (R)x.x, (R)x.y);
+    cmul_acc(o1r, o1i, d_r, d_i, (R)y.x, (R)y.y);
+    C r0, r1;
+    r0.x = o0r;
+    r0.y = o0i;
+    r1.x = o1r;
+    r1.y = o1i;
+    psi[i0] = r0;
+    psi[i1] = r1;
+  }
+}
+
+// diag(d0, d1) on bit t where the control bits are all 1.
+template <typename R>
+__global__ void k_diag1(typename CT<R>::T* __restrict__ psi, uint64_t dim, int t, uint64_t cmask,
+                        double2 d0, double2 d1) {
+  using C = typename CT<R>::T;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    if ((i & cmask) != cmask) continue;
+    const double2 d = ((i >> t) & 1) ? d1 : d0;
+    const C x = psi[i];
+    C y;
+    y.x = (R)d.x * x.x - (R)d.y * x.y;
+    y.y = (R)d.x * x.y + (R)d.y * x.x;
+    psi[i] = y;
+  }
+}
+
+// Dense 2^k x 2^k on k target bits (+ controls); one thread per group.
+template <typename R>
+__global__ void k_densek(typename CT<R>::T* __restrict__ psi, typename CT<R>::T* __restrict__ tmp,
+                         uint64_t ngroups, DenseKArgs a, const double2* __restrict__ M,
+                         const uint64_t* __restrict__ off) {
+  using C = typename CT<R>::T;
+  const int dim = 1 << a.k;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t base = insert_zero_bits(g, a.sorted_bits, a.k);
+    if ((base & a.cmask) != a.cmask) continue;
+    C* out = tmp + ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * (uint64_t)dim;
+    for (int r = 0; r < dim; ++r) {
+      R ar = 0, ai = 0;
+      for (int j = 0; j < dim; ++j) {
+        const double2 m = M[(uint64_t)r * dim + j];
+        const C v = psi[base + off[j]];
+        cmul_acc(ar, ai, (R)m.x, (R)m.y, (R)v.x, (R)v.y);
+      }
+      C o;
+      o.x = ar;
+      o.y = ai;
+      out[r] = o;
+    }
+    for (int r = 0; r < dim; ++r) psi[base + off[r]] = out[r];
+  }
+}
+
+// 2^k diagonal phases over k bits (+ controls).
+template <typename R>
+__global__ void k_diagk(typename CT<R>::T* __restrict__ psi, uint64_t dim, DiagKArgs a,
+                        const double2* __restrict__ D) {
+  using C = typename CT<R>::T;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    if ((i & a.cmask) != a.cmask) continue;
+    int l = 0;
+    for (int q = 0; q < a.k; ++q) l = (l << 1) | (int)((i >> a.bits[q]) & 1);
+    const double2 d = D[l];
+    const C x = psi[i];
+    C y;
+    y.x = (R)d.x * x.x - (R)d.y * x.y;
+    y.y = (R)d.x * x.y + (R)d.y * x.x;
+    psi[i] = y;
+  }
+}
+
+// =====================================================================================
+// Fused tile sweep (interpreter). See planner.cpp for the program it runs.
+// =====================================================================================
+
+template <typename R, int RB>
+struct Regs {
+  typename CT<R>::T v[1 << RB];
+};
+
+// Applies a 2x2 on register slot J to every register pair whose control predicate holds.
+template <typename R, int RB, int J, int FORM, bool CTRL>
+__device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const double2* __restrict__ u, uint32_t ns,
+                                        bool lane_ok) {
+  const R u00r = (R)u[0].x, u00i = (R)u[0].y, u01r = (R)u[1].x, u01i = (R)u[1].y;
+  const R u10r = (R)u[2].x, u10i = (R)u[2].y, u11r = (R)u[3].x, u11i = (R)u[3].y;
+#pragma unroll
+  for (int m0 = 0; m0 < (1 << RB); ++m0) {
+    if (m0 & (1 << J)) continue;
+    const int m1 = m0 | (1 << J);
+    if (CTRL) {
+      if (!lane_ok || ((uint32_t)m0 & ns) != ns) continue;
+    }
+    const R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
+    R o0r, o0i, o1r, o1i;
+    if (FORM == 3) {  // X: swap
+      o0r = yr; o0i = yi; o1r = xr; o1i = xi;
+    } else if (FORM == 1) {  // anti-diagonal
+      o0r = u01r * yr - u01i * yi;
+      o0i = u01r * yi + u01i * yr;
+      o1r = u10r * xr - u10i * xi;
+      o1i = u10r * xi + u10i * xr;
+    } else if (FORM == 2) {  // real
+      o0r = fma(u00r, xr, u01r * yr);
+      o0i = fma(u00r, xi, u01r * yi);
+      o1r = fma(u10r, xr, u11r * yr);
+      o1i = fma(u10r, xi, u11r * yi);
+    } else {
+      o0r = u00r * xr; o0i = u00r * xi;
+      o0r = fma(-u00i, xi, o0r); o0i = fma(u00i, xr, o0i);
+      o0r = fma(u01r, yr, o0r); o0i = fma(u01r, yi, o0i);
+      o0r = fma(-u01i, yi, o0r); o0i = fma(u01i, yr, o0i);
+      o1r = u10r * xr; o1i = u10r * xi;
+      o1r = fma(-u10i, xi, o1r); o1i = fma(u10i, xr, o1i);
+      o1r = fma(u11r, yr, o1r); o1i = fma(u11r, yi, o1i);
+      o1r = fma(-u11i, yi, o1r); o1i = fma(u11i, yr, o1i);
+    }
+    a.v[m0].x = o0r; a.v[m0].y = o0i;
+    a.v[m1].x = o1r; a.v[m1].y = o1i;
+  }
+}
+
+template <typename R, int RB, int J, bool CTRL>
+__device__ __forceinline__ void u1_dispatch_form(Regs<R, RB>& a, const double2* u, uint32_t form,
+                                                 uint32_t ns, bool lane_ok) {
+  switch (form) {
+    case 3: u1_slot<R, RB, J, 3, CTRL>(a, u, ns, lane_ok); break;
+    case 1: u1_slot<R, RB, J, 1, CTRL>(a, u, ns, lane_ok); break;
+    case 2: u1_slot<R, RB, J, 2, CTRL>(a, u, ns, lane_ok); break;
+    default: u1_slot<R, RB, J, 0, CTRL>(a, u, ns, lane_ok); break;
+  }
+}
+
+template <typename R, int RB, bool CTRL>
+__device__ __forceinline__ void u1_dispatch(Regs<R, RB>& a, uint32_t slot, const double2* u,
+                                            uint32_t form, uint32_t ns, bool lane_ok) {
+  switch (slot) {
+    case 0: u1_dispatch_form<R, RB, 0, CTRL>(a, u, form, ns, lane_ok); break;
+    case 1: if constexpr (RB > 1) u1_dispatch_form<R, RB, 1, CTRL>(a, u, form, ns, lane_ok); break;
+    case 2: if constexpr (RB > 2) u1_dispatch_form<R, RB, 2, CTRL>(a, u, form, ns, lane_ok); break;
+    case 3: if constexpr (RB > 3) u1_dispatch_form<R, RB, 3, CTRL>(a, u, form, ns, lane_ok); break;
+    case 4: if constexpr (RB > 4) u1_dispatch_form<R, RB, 4, CTRL>(a, u, form, ns, lane_ok); break;
+    default: break;
+  }
+}
+
+// diag(d0,d1) where the bit is register slot J (compile-time) or a per-thread bit (J < 0).
+template <typename R, int RB, int J, bool ONLY1>
+__device__ __forceinline__ void d1_apply(Regs<R, RB>& a, const double2* __restrict__ d, bool lane_bit,
+                                         uint32_t ns, bool lane_ok) {
+  const R d0r = (R)d[0].x, d0i = (R)d[0].y, d1r = (R)d[1].x, d1i = (R)d[1].y;
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) {
+    if (((uint32_t)m & ns) != ns) continue;
+    bool bit;
+    if (J >= 0) bit = (m >> (J >= 0 ? J : 0)) & 1;
+    else bit = lane_bit;
+    if (J >= 0 && ONLY1 && !bit) continue;
+    const bool act = lane_ok && (!ONLY1 || bit);
+    const R fr = bit ? d1r : d0r, fi = bit ? d1i : d0i;
+    const R xr = a.v[m].x, xi = a.v[m].y;
+    const R yr = fma(fr, xr, -fi * xi);
+    const R yi = fma(fr, xi, fi * xr);
+    a.v[m].x = act ? yr : xr;
+    a.v[m].y = act ? yi : xi;
+  }
+}
+
+template <typename R, int RB, bool ONLY1>
+__device__ __forceinline__ void d1_dispatch(Regs<R, RB>& a, uint32_t slot, const double2* d,
+                                            bool lane_bit, uint32_t ns, bool lane_ok) {
+  switch (slot) {
+    case 0: d1_apply<R, RB, 0, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+    case 1: if constexpr (RB > 1) d1_apply<R, RB, 1, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+    case 2: if constexpr (RB > 2) d1_apply<R, RB, 2, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+    case 3: if constexpr (RB > 3) d1_apply<R, RB, 3, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+    case 4: if constexpr (RB > 4) d1_apply<R, RB, 4, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+    default: d1_apply<R, RB, -1, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
+  }
+}
+
+template <typename R, int K, int RB>
+struct TileCtx {
+  static constexpr int NT = K - RB;
+  uint64_t g0;        // global index of register 0 of this thread
+  uint32_t l0;        // tile-local index of register 0
+  uint64_t gb[RB];    // global bit of each register slot
+  uint32_t lb[RB];    // tile-local bit of each register slot
+  uint64_t regmask;   // OR of gb
+
+  __device__ __forceinline__ void setup(const uint8_t* __restrict__ mp, const TileArgs& a,
+                                        uint64_t base, uint32_t tid) {
+    g0 = base;
+    l0 = 0;
+#pragma unroll
+    for (int s = 0; s < NT; ++s) {
+      const int p = mp[s];
+      if ((tid >> s) & 1) {
+        g0 |= 1ull << a.tbits[p];
+        l0 |= 1u << p;
+      }
+    }
+    regmask = 0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j) {
+      const int p = mp[NT + j];
+      gb[j] = 1ull << a.tbits[p];
+      lb[j] = 1u << p;
+      regmask |= gb[j];
+    }
+  }
+  __device__ __forceinline__ uint64_t gidx(int m) const {
+    uint64_t x = g0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if ((m >> j) & 1) x |= gb[j];
+    return x;
+  }
+  __device__ __forceinline__ uint32_t lidx(int m) const {
+    uint32_t x = l0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if ((m >> j) & 1) x |= lb[j];
+    return x;
+  }
+  // Control predicate split: lane_ok = the thread/base part cannot fail; ns = register slots
+  // that must be 1.
+  __device__ __forceinline__ void controls(uint64_t cmask, uint32_t& ns, bool& lane_ok) const {
+    const uint64_t need = cmask & ~g0;
+    lane_ok = (need & ~regmask) == 0;
+    ns = 0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j)
+      if (need & gb[j]) ns |= 1u << j;
+  }
+};
+
+template <typename R, int K, int RB>
+__global__ __launch_bounds__(1 << (K - RB)) void k_tile(typename CT<R>::T* __restrict__ psi, TileArgs a) {
+  using C = typename CT<R>::T;
+  __shared__ C lds[1 << K];
+  const uint32_t tid = threadIdx.x;
+  uint64_t base = 0;
+  {
+    const uint64_t b = blockIdx.x;
+    for (int i = 0; i < a.nnontile; ++i) base |= ((b >> i) & 1ull) << a.nontile[i];
+  }
+  TileCtx<R, K, RB> ctx;
+  ctx.setup(a.maps, a, base, tid);
+  Regs<R, RB> r;
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) r.v[m] = psi[ctx.gidx(m)];
+
+  for (uint32_t o = 0; o < a.nops; ++o) {
+    const TileOpDev op = a.ops[o];
+    switch (op.code) {
+      case TC_U1: {
+        const double2* u = a.coefs + op.coef;
+        if (op.cmask == 0) {
+          u1_dispatch<R, RB, false>(r, op.a, u, op.b, 0, true);
+        } else {
+          uint32_t ns;
+          bool ok;
+          ctx.controls(op.cmask, ns, ok);
+          u1_dispatch<R, RB, true>(r, op.a, u, op.b, ns, ok);
+        }
+        break;
+      }
+      case TC_D1: {
+        const double2* d = a.coefs + op.coef;
+        uint32_t ns = 0;
+        bool ok = true;
+        if (op.cmask) ctx.controls(op.cmask, ns, ok);
+        const bool lane_bit = (ctx.g0 >> op.a) & 1;
+        if (op.c) d1_dispatch<R, RB, true>(r, op.b, d, lane_bit, ns, ok);
+        else d1_dispatch<R, RB, false>(r, op.b, d, lane_bit, ns, ok);
+        break;
+      }
+      case TC_REMAP: {
+        __syncthreads();
+#pragma unroll
+        for (int m = 0; m < (1 << RB); ++m) lds[ctx.lidx(m)] = r.v[m];
+        __syncthreads();
+        ctx.setup(a.maps + (uint32_t)K * op.a, a, base, tid);
+#pragma unroll
+        for (int m = 0; m < (1 << RB); ++m) r.v[m] = lds[ctx.lidx(m)];
+        break;
+      }
+      case TC_SWAPSLOTS: {
+        ctx.setup(a.maps + (uint32_t)K * op.a, a, base, tid);
+        break;
+      }
+      default:
+        break;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) psi[ctx.gidx(m)] = r.v[m];
+}
+
+// =====================================================================================
+// Sampling (SuperPosition::measure, super_positions.rs:332-342) with injected draws.
+// =====================================================================================
+
+// Exact left fold: one workgroup streams the probabilities through LDS; thread 0 adds them
+// in index order (re*re + im*im, no fused operations) and assigns the sorted draws.
+template <typename R>
+__global__ __launch_bounds__(1024) void k_sample_fold(const typename CT<R>::T* __restrict__ psi,
+                                                      uint64_t dim, const double* __restrict__ u,
+                                                      uint64_t shots, uint64_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int CH = 4096;
+  __shared__ double p[CH];
+  __shared__ int done;
+  double cum = 0.0;
+  uint64_t d = 0;
+  double ud = shots ? u[0] : 0.0;
+  if (threadIdx.x == 0) done = (shots == 0);
+  __syncthreads();
+  for (uint64_t base = 0; base < dim && !done; base += CH) {
+    for (int q = threadIdx.x; q < CH; q += blockDim.x) {
+      const uint64_t i = base + q;
+      double v = 0.0;
+      if (i < dim) {
+        const double re = (double)psi[i].x, im = (double)psi[i].y;
+        v = re * re + im * im;
+      }
+      p[q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t cnt = (dim - base < (uint64_t)CH) ? (dim - base) : (uint64_t)CH;
+      for (uint64_t q = 0; q < cnt; ++q) {
+        cum = cum + p[q];
+        while (d < shots && ud < cum) {
+          out[d] = base + q;
+          ++d;
+          ud = (d < shots) ? u[d] : 0.0;
+        }
+        if (d >= shots) break;
+      }
+      done = (d >= shots);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (; d < shots; ++d) out[d] = ~0ull;
+}
+
+// Parallel path, step 1: probability mass of each block of 2^bb amplitudes (f64).
+template <typename R>
+__global__ __launch_bounds__(256) void k_block_mass(const typename CT<R>::T* __restrict__ psi,
+                                                    uint64_t dim, int bb, double* __restrict__ mass) {
+  __shared__ double red[256];
+  const uint64_t b0 = (uint64_t)blockIdx.x << bb;
+  const uint64_t len = 1ull << bb;
+  double s = 0.0;
+  for (uint64_t q = threadIdx.x; q < len; q += blockDim.x) {
+    const uint64_t i = b0 + q;
+    if (i < dim) {
+      const double re = (double)psi[i].x, im = (double)psi[i].y;
+      s += re * re + im * im;
+    }
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) mass[blockIdx.x] = red[0];
+}
+
+// Parallel path, step 2: one workgroup per visited block folds it from its start value.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sample_blocks(const typename CT<R>::T* __restrict__ psi,
+                                                       uint64_t dim, int bb, const BlockJob* __restrict__ jobs,
+                                                       const double* __restrict__ u, uint64_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int CH = 2048;
+  __shared__ double p[CH];
+  __shared__ uint64_t dd;
+  const BlockJob job = jobs[blockIdx.x];
+  const uint64_t b0 = job.block << bb;
+  const uint64_t len = 1ull << bb;
+  double cum = job.start;
+  uint64_t d = job.first;
+  for (uint64_t c0 = 0; c0 < len && d < job.last; c0 += CH) {
+    for (int q = threadIdx.x; q < CH; q += blockDim.x) {
+      const uint64_t i = b0 + c0 + q;
+      double v = 0.0;
+      if (c0 + q < len && i < dim) {
+        const double re = (double)psi[i].x, im = (double)psi[i].y;
+        v = re * re + im * im;
+      }
+      p[q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int q = 0; q < CH && d < job.last; ++q) {
+        cum = cum + p[q];
+        while (d < job.last && u[d] < cum) {
+          out[d] = b0 + c0 + q;
+          ++d;
+        }
+      }
+    }
+    __syncthreads();
+    // broadcast progress
+    if (threadIdx.x == 0) dd = d;
+    __syncthreads();
+    d = dd;
+  }
+  // draws that the block did not resolve (rounding at the block end) take its last index
+  if (threadIdx.x == 0)
+    for (; d < job.last; ++d) out[d] = b0 + len - 1;
+}
+
+// =====================================================================================
+// Reductions (f64 accumulation; complex64 promoted before multiplying)
+// =====================================================================================
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_norm_partial(const typename CT<R>::T* __restrict__ psi,
+                                                      uint64_t dim, double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0, comp = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
+    const double re = (double)psi[i].x, im = (double)psi[i].y;
+    const double v = re * re + im * im;
+    const double y = v - comp;
+    const double t = s + y;
+    comp = (t - s) - y;
+    s = t;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// sum_j conj(phi_{j ^ x}) * s_z(j) * psi_j  (real and imaginary parts), s_z = (-1)^popcount(j & z).
+// With phi == psi this is <psi|P|psi> / i^ny.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pauli_partial(const typename CT<R>::T* __restrict__ phi,
+                                                       const typename CT<R>::T* __restrict__ psi,
+                                                       uint64_t dim, uint64_t xm, uint64_t zm,
+                                                       double2* __restrict__ part) {
+  __shared__ double rr[256], ri[256];
+  double sr = 0.0, si = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride) {
+    const uint64_t i = j ^ xm;
+    const double ar = (double)phi[i].x, ai = -(double)phi[i].y;
+    const double br = (double)psi[j].x, bi = (double)psi[j].y;
+    const double sg = (__popcll(j & zm) & 1) ? -1.0 : 1.0;
+    sr += sg * (ar * br - ai * bi);
+    si += sg * (ar * bi + ai * br);
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rr[threadIdx.x] += rr[threadIdx.x + w];
+      ri[threadIdx.x] += ri[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
+}
+
+// Diagonal Pauli terms together: sum_j |psi_j|^2 * sum_t c_t (-1)^popcount(j & z_t).
+template <typename R>
+__global__ __launch_bounds__(256) void k_zdiag_partial(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                       const uint64_t* __restrict__ zmasks,
+                                                       const double* __restrict__ coeffs, int nterms,
+                                                       double* __restrict__ part) {
+  __shared__ double red[256];
+  double s = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride) {
+    const double re = (double)psi[j].x, im = (double)psi[j].y;
+    const double p = re * re + im * im;
+    double w = 0.0;
+    for (int t = 0; t < nterms; ++t) w += (__popcll(j & zmasks[t]) & 1) ? -coeffs[t] : coeffs[t];
+    s += p * w;
+  }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// lam (+)= c * i^ny * (-1)^popcount(j&z) psi_{j^x}   evaluated at output index j' = j ^ x:
+// (P psi)_o = i^ny (-1)^popcount((o ^ x) & z) psi_{o ^ x}.
+template <typename R>
+__global__ void k_pauli_apply(typename CT<R>::T* __restrict__ lam, const typename CT<R>::T* __restrict__ psi,
+                              uint64_t dim, uint64_t xm, uint64_t zm, double cr, double ci, int accumulate) {
+  using C = typename CT<R>::T;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < dim; o += stride) {
+    const uint64_t j = o ^ xm;
+    const double sg = (__popcll(j & zm) & 1) ? -1.0 : 1.0;
+    const double vr = (double)psi[j].x * sg, vi = (double)psi[j].y * sg;
+    double yr = cr * vr - ci * vi, yi = cr * vi + ci * vr;
+    if (accumulate) {
+      yr += (double)lam[o].x;
+      yi += (double)lam[o].y;
+    }
+    C w;
+    w.x = (R)yr;
+    w.y = (R)yi;
+    lam[o] = w;
+  }
+}
+
+// Gradient term of one rotation gate: sum_j conj(lam_j) (G psi)_j for G in {X, Y, Z, I} on bit t.
+// Returns partial (re, im). Y: (Y psi)_j = -i (-1)^{bit_t(j)} psi_{j^t}; Z: (-1)^{bit_t(j)} psi_j.
+template <typename R>
+__global__ __launch_bounds__(256) void k_gen_partial(const typename CT<R>::T* __restrict__ lam,
+                                                     const typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                     int t, int gen, double2* __restrict__ part) {
+  __shared__ double rr[256], ri[256];
+  double sr = 0.0, si = 0.0;
+  const uint64_t tb = 1ull << t;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride) {
+    double gr, gi;
+    if (gen == 1) {
+      gr = (double)psi[j ^ tb].x;
+      gi = (double)psi[j ^ tb].y;
+    } else if (gen == 2) {
+      const double s = (j & tb) ? -1.0 : 1.0;
+      const double pr = (double)psi[j ^ tb].x * s, pi = (double)psi[j ^ tb].y * s;
+      gr = pi;   // -i * (pr + i pi) = pi - i pr
+      gi = -pr;
+    } else if (gen == 3) {
+      const double s = (j & tb) ? -1.0 : 1.0;
+      gr = (double)psi[j].x * s;
+      gi = (double)psi[j].y * s;
+    } else {
+      gr = (double)psi[j].x;
+      gi = (double)psi[j].y;
+    }
+    const double lr = (double)lam[j].x, li = -(double)lam[j].y;
+    sr += lr * gr - li * gi;
+    si += lr * gi + li * gr;
+  }
+  rr[threadIdx.x] = sr;
+  ri[threadIdx.x] = si;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      rr[threadIdx.x] += rr[threadIdx.x + w];
+      ri[threadIdx.x] += ri[threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
+}
+
+// complex64 <-> binary64 conversion for state upload/download
+__global__ void k_f64_to_c64(float2* __restrict__ dst, const double2* __restrict__ src, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = make_float2((float)src[i].x, (float)src[i].y);
+}
+__global__ void k_c64_to_f64(double2* __restrict__ dst, const float2* __restrict__ src, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    dst[i] = make_double2((double)src[i].x, (double)src[i].y);
+}
+
+}  // namespace dev
+
+// =====================================================================================
+// Host launchers
+// =====================================================================================
+
+namespace {
+inline unsigned grid_for(uint64_t work, int block, unsigned cap = 1u << 20) {
+  uint64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > cap) g = cap;
+  return (unsigned)g;
+}
+}  // namespace
+
+int tile_max_bits(int prec) { return prec == 0 ? 12 : 13; }
+int tile_reg_bits(int K) { return K < 4 ? K : 4; }
+
+hipError_t launch_init_basis(int prec, void* psi, uint64_t dim, uint64_t one_at, hipStream_t s) {
+  const unsigned g = grid_for(dim, 256, 1u << 16);
+  if (prec == 0)
+    dev::k_init_basis<double><<<g, 256, 0, s>>>((double2*)psi, dim, one_at);
+  else
+    dev::k_init_basis<float><<<g, 256, 0, s>>>((float2*)psi, dim, one_at);
+  return hipGetLastError();
+}
+
+hipError_t launch_exact(int prec, void* psi, uint64_t ngroups, const ExactArgs& a, const double2* M,
+                        hipStream_t s) {
+  const unsigned g = grid_for(ngroups, 256, 1u << 16);
+#define QR_EXACT_CASE(KR)                                                                   \
+  case KR:                                                                                  \
+    if (prec == 0)                                                                          \
+      dev::k_exact<double, KR><<<g, 256, 0, s>>>((double2*)psi, ngroups, a, M);             \
+    else                                                                                    \
+      dev::k_exact<float, KR><<<g, 256, 0, s>>>((float2*)psi, ngroups, a, M);               \
+    break;
+  switch (a.K) {
+    QR_EXACT_CASE(1)
+    QR_EXACT_CASE(2)
+    QR_EXACT_CASE(3)
+    QR_EXACT_CASE(4)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef QR_EXACT_CASE
+  return hipGetLastError();
+}
+
+hipError_t launch_exact_wide(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
+                             const ExactWideArgs& a, const double2* M, const uint64_t* off,
+                             const uint32_t* order, const uint8_t* flags, hipStream_t s) {
+  // one thread per tmp slot: grid * block == tmp_groups
+  const int block = tmp_groups >= 256 ? 256 : (int)tmp_groups;
+  const unsigned g = (unsigned)(tmp_groups / block);
+  if (prec == 0)
+    dev::k_exact_wide<double><<<g, block, 0, s>>>((double2*)psi, (double2*)tmp, ngroups, a, M, off, order, flags);
+  else
+    dev::k_exact_wide<float><<<g, block, 0, s>>>((float2*)psi, (float2*)tmp, ngroups, a, M, off, order, flags);
+  return hipGetLastError();
+}
+
+hipError_t launch_dense1(int prec, void* psi, uint64_t npairs, int t, uint64_t cmask, const double2* u,
+                         hipStream_t s) {
+  const unsigned g = grid_for(npairs, 256, 1u << 18);
+  if (prec == 0)
+    dev::k_dense1<double><<<g, 256, 0, s>>>((double2*)psi, npairs, t, cmask, u[0], u[1], u[2], u[3]);
+  else
+    dev::k_dense1<float><<<g, 256, 0, s>>>((float2*)psi, npairs, t, cmask, u[0], u[1], u[2], u[3]);
+  return hipGetLastError();
+}
+
+hipError_t launch_diag1(int prec, void* psi, uint64_t dim, int t, uint64_t cmask, double2 d0, double2 d1,
+                        hipStream_t s) {
+  const unsigned g = grid_for(dim, 256, 1u << 18);
+  if (prec == 0)
+    dev::k_diag1<double><<<g, 256, 0, s>>>((double2*)psi, dim, t, cmask, d0, d1);
+  else
+    dev::k_diag1<float><<<g, 256, 0, s>>>((float2*)psi, dim, t, cmask, d0, d1);
+  return hipGetLastError();
+}
+
+hipError_t launch_densek(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
+                         const DenseKArgs& a, const double2* M, const uint64_t* off, hipStream_t s) {
+  const int block = tmp_groups >= 256 ? 256 : (int)tmp_groups;
+  const unsigned g = (unsigned)(tmp_groups / block);
+  if (prec == 0)
+    dev::k_densek<double><<<g, block, 0, s>>>((double2*)psi, (double2*)tmp, ngroups, a, M, off);
+  else
+    dev::k_densek<float><<<g, block, 0, s>>>((float2*)psi, (float2*)tmp, ngroups, a, M, off);
+  return hipGetLastError();
+}
+
+hipError_t launch_diagk(int prec, void* psi, uint64_t dim, const DiagKArgs& a, const double2* D,
+                        hipStream_t s) {
+  const unsigned g = grid_for(dim, 256, 1u << 18);
+  if (prec == 0)
+    dev::k_diagk<double><<<g, 256, 0, s>>>((double2*)psi, dim, a, D);
+  else
+    dev::k_diagk<float><<<g, 256, 0, s>>>((float2*)psi, dim, a, D);
+  return hipGetLastError();
+}
+
+hipError_t launch_tile(int prec, int K, void* psi, uint64_t ntiles, const TileArgs& a, hipStream_t s) {
+  if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
+  const unsigned g = (unsigned)ntiles;
+#define QR_TILE_CASE(KK)                                                                       \
+  case KK: {                                                                                   \
+    constexpr int RBv = KK < 4 ? KK : 4;                                                       \
+    constexpr int threads = 1 << (KK - RBv);                                                   \
+    if (prec == 0) {                                                                           \
+      if constexpr (KK <= 12) dev::k_tile<double, KK, RBv><<<g, threads, 0, s>>>((double2*)psi, a); \
+      else return hipErrorInvalidValue;                                                        \
+    } else {                                                                                   \
+      dev::k_tile<float, KK, RBv><<<g, threads, 0, s>>>((float2*)psi, a);                      \
+    }                                                                                          \
+    break;                                                                                     \
+  }
+  switch (K) {
+    QR_TILE_CASE(1)
+    QR_TILE_CASE(2)
+    QR_TILE_CASE(3)
+    QR_TILE_CASE(4)
+    QR_TILE_CASE(5)
+    QR_TILE_CASE(6)
+    QR_TILE_CASE(7)
+    QR_TILE_CASE(8)
+    QR_TILE_CASE(9)
+    QR_TILE_CASE(10)
+    QR_TILE_CASE(11)
+    QR_TILE_CASE(12)
+    QR_TILE_CASE(13)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef QR_TILE_CASE
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const double* u, uint64_t shots,
+                              uint64_t* out, hipStream_t s) {
+  if (prec == 0)
+    dev::k_sample_fold<double><<<1, 1024, 0, s>>>((const double2*)psi, dim, u, shots, out);
+  else
+    dev::k_sample_fold<float><<<1, 1024, 0, s>>>((const float2*)psi, dim, u, shots, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_block_mass(int prec, const void* psi, uint64_t dim, int bb, double* mass, hipStream_t s) {
+  const uint64_t nb = (dim + (1ull << bb) - 1) >> bb;
+  if (prec == 0)
+    dev::k_block_mass<double><<<(unsigned)nb, 256, 0, s>>>((const double2*)psi, dim, bb, mass);
+  else
+    dev::k_block_mass<float><<<(unsigned)nb, 256, 0, s>>>((const float2*)psi, dim, bb, mass);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_blocks(int prec, const void* psi, uint64_t dim, int bb, const BlockJob* jobs,
+                                uint64_t njobs, const double* u, uint64_t* out, hipStream_t s) {
+  if (njobs == 0) return hipSuccess;
+  if (prec == 0)
+    dev::k_sample_blocks<double><<<(unsigned)njobs, 256, 0, s>>>((const double2*)psi, dim, bb, jobs, u, out);
+  else
+    dev::k_sample_blocks<float><<<(unsigned)njobs, 256, 0, s>>>((const float2*)psi, dim, bb, jobs, u, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_norm_partial(int prec, const void* psi, uint64_t dim, double* part, int nblocks,
+                               hipStream_t s) {
+  if (prec == 0)
+    dev::k_norm_partial<double><<<nblocks, 256, 0, s>>>((const double2*)psi, dim, part);
+  else
+    dev::k_norm_partial<float><<<nblocks, 256, 0, s>>>((const float2*)psi, dim, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pauli_partial(int prec, const void* phi, const void* psi, uint64_t dim, uint64_t xm,
+                                uint64_t zm, double2* part, int nblocks, hipStream_t s) {
+  if (prec == 0)
+    dev::k_pauli_partial<double><<<nblocks, 256, 0, s>>>((const double2*)phi, (const double2*)psi, dim, xm, zm, part);
+  else
+    dev::k_pauli_partial<float><<<nblocks, 256, 0, s>>>((const float2*)phi, (const float2*)psi, dim, xm, zm, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const uint64_t* zmasks,
+                                const double* coeffs, int nterms, double* part, int nblocks,
+                                hipStream_t s) {
+  if (prec == 0)
+    dev::k_zdiag_partial<double><<<nblocks, 256, 0, s>>>((const double2*)psi, dim, zmasks, coeffs, nterms, part);
+  else
+    dev::k_zdiag_partial<float><<<nblocks, 256, 0, s>>>((const float2*)psi, dim, zmasks, coeffs, nterms, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, uint64_t zm,
+                              double cr, double ci, int accumulate, hipStream_t s) {
+  const unsigned g = grid_for(dim, 256, 1u << 18);
+  if (prec == 0)
+    dev::k_pauli_apply<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, xm, zm, cr, ci, accumulate);
+  else
+    dev::k_pauli_apply<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, xm, zm, cr, ci, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
+                              double2* part, int nblocks, hipStream_t s) {
+  if (prec == 0)
+    dev::k_gen_partial<double><<<nblocks, 256, 0, s>>>((const double2*)lam, (const double2*)psi, dim, t, gen, part);
+  else
+    dev::k_gen_partial<float><<<nblocks, 256, 0, s>>>((const float2*)lam, (const float2*)psi, dim, t, gen, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_f64_to_c64(float2* dst, const double2* src, uint64_t n, hipStream_t s) {
+  dev::k_f64_to_c64<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_c64_to_f64(double2* dst, const float2* src, uint64_t n, hipStream_t s) {
+  dev::k_c64_to_f64<<<grid_for(n, 256, 1u << 16), 256, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
+}  // namespace qr

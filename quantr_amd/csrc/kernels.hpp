@@ -1,0 +1,108 @@
+// kernels.hpp — argument blocks and host launchers of the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace qr {
+
+// Exact per-gate sweep (K <= 4 acting wires, amplitudes held in registers).
+struct ExactArgs {
+  int K;
+  uint8_t sorted_bits[8];  // acting bits ascending
+  uint64_t off[16];        // offset of local basis state j (MSB-first local index)
+  uint8_t order[16];       // local states in ascending global offset order
+  uint32_t some;           // bit j: input j contributes
+  uint32_t keep;           // bit r: output r keeps its input (custom None)
+};
+
+struct ExactWideArgs {
+  int K;
+  uint8_t sorted_bits[20];
+};
+
+struct DenseKArgs {
+  int k;
+  uint8_t sorted_bits[20];
+  uint64_t cmask;
+};
+
+struct DiagKArgs {
+  int k;
+  uint8_t bits[20];  // local MSB first
+  uint64_t cmask;
+};
+
+// Tile op codes (device interpreter).
+enum TileCode : uint32_t {
+  TC_END = 0,
+  TC_REMAP = 1,      // a = index of the new slot map (data moves through LDS)
+  TC_U1 = 2,         // a = register slot, coef -> 4 complex, b = form
+  TC_D1 = 3,         // a = global bit, b = register slot or 0xFF, coef -> 2 complex, c = only |1>
+  TC_SWAPSLOTS = 4,  // a = index of the new slot map (relabel, no data movement)
+};
+
+struct TileOpDev {
+  uint32_t code, a;
+  uint64_t cmask;
+  uint32_t coef, b, c, d;
+};
+static_assert(sizeof(TileOpDev) == 32, "TileOpDev layout");
+
+struct TileArgs {
+  const TileOpDev* ops;
+  const uint8_t* maps;
+  const double2* coefs;
+  uint32_t nops;
+  int nnontile;
+  uint8_t tbits[16];
+  uint8_t nontile[48];
+};
+
+struct BlockJob {
+  uint64_t block;
+  double start;
+  uint64_t first, last;
+};
+
+// Tile kernel variants available: tile bits K and register bits RB per precision.
+int tile_max_bits(int prec);  // 12 for complex128, 13 for complex64
+int tile_reg_bits(int K);
+
+// ---- launchers (prec: 0 = complex128, 1 = complex64) ----
+hipError_t launch_init_basis(int prec, void* psi, uint64_t dim, uint64_t one_at, hipStream_t s);
+hipError_t launch_exact(int prec, void* psi, uint64_t ngroups, const ExactArgs& a, const double2* M,
+                        hipStream_t s);
+hipError_t launch_exact_wide(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
+                             const ExactWideArgs& a, const double2* M, const uint64_t* off,
+                             const uint32_t* order, const uint8_t* flags, hipStream_t s);
+hipError_t launch_dense1(int prec, void* psi, uint64_t npairs, int t, uint64_t cmask, const double2* u,
+                         hipStream_t s);
+hipError_t launch_diag1(int prec, void* psi, uint64_t dim, int t, uint64_t cmask, double2 d0, double2 d1,
+                        hipStream_t s);
+hipError_t launch_densek(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
+                         const DenseKArgs& a, const double2* M, const uint64_t* off, hipStream_t s);
+hipError_t launch_diagk(int prec, void* psi, uint64_t dim, const DiagKArgs& a, const double2* D,
+                        hipStream_t s);
+hipError_t launch_tile(int prec, int K, void* psi, uint64_t ntiles, const TileArgs& a, hipStream_t s);
+hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const double* u, uint64_t shots,
+                              uint64_t* out, hipStream_t s);
+hipError_t launch_block_mass(int prec, const void* psi, uint64_t dim, int bb, double* mass, hipStream_t s);
+hipError_t launch_sample_blocks(int prec, const void* psi, uint64_t dim, int bb, const BlockJob* jobs,
+                                uint64_t njobs, const double* u, uint64_t* out, hipStream_t s);
+hipError_t launch_norm_partial(int prec, const void* psi, uint64_t dim, double* part, int nblocks,
+                               hipStream_t s);
+hipError_t launch_pauli_partial(int prec, const void* phi, const void* psi, uint64_t dim, uint64_t xm,
+                                uint64_t zm, double2* part, int nblocks, hipStream_t s);
+hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const uint64_t* zmasks,
+                                const double* coeffs, int nterms, double* part, int nblocks,
+                                hipStream_t s);
+hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, uint64_t zm,
+                              double cr, double ci, int accumulate, hipStream_t s);
+hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
+                              double2* part, int nblocks, hipStream_t s);
+hipError_t launch_f64_to_c64(float2* dst, const double2* src, uint64_t n, hipStream_t s);
+hipError_t launch_c64_to_f64(double2* dst, const float2* src, uint64_t n, hipStream_t s);
+
+}  // namespace qr

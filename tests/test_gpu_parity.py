@@ -1,0 +1,261 @@
+"""GPU parity: the HIP engine through the C ABI against the oracle and the golden vectors.
+
+Bars: the unfused reference-order path (fused=0) is bitwise equal to the oracle; the fused
+path is within 1e-12 max amplitude error (complex128); complex64 within 2e-5; bins identical
+for the same injected draws.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+from quantr_amd import _capi
+from quantr_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+TOL_FUSED = 1e-12
+TOL_C64 = 2e-5
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if _capi.device_count() < 1:
+        pytest.fail("no GPU visible: the HIP engine cannot run")
+
+
+def run_gpu(n, program, fused=1, precision=_capi.QR_C128, register=None, tile_bits=0):
+    st = _capi.State(n, precision)
+    st.set_options(fused=fused, tile_bits=tile_bits)
+    if register is not None:
+        st.set_amplitudes(register)
+    st.apply(program)
+    out = st.amplitudes()
+    st.close()
+    return out
+
+
+def nested_gpu(k, program, reg):
+    return run_gpu(k, program, fused=0, register=reg)
+
+
+@pytest.mark.parametrize("fused", [0, 1])
+def test_golden_vectors_gpu(fused):
+    d = H.golden()
+    for case in d["cases"]:
+        if "expected" not in case:
+            continue
+        say("golden", case["name"], "fused", fused)
+        p = H.program_from_fixture(case["program"], d["tables"], nested_gpu)
+        reg = H.cvec(case["register"]) if "register" in case else None
+        got = run_gpu(case["n"], p, fused=fused, register=reg)
+        want = H.cvec(case["expected"])
+        assert np.max(np.abs(got - want)) < 1e-12, case["name"]
+        orc = H.oracle_simulate(case["n"], p, register=reg)
+        if fused == 0:
+            assert np.array_equal(got, orc), case["name"]
+        else:
+            assert np.max(np.abs(got - orc)) < TOL_FUSED, case["name"]
+
+
+def test_random_circuits_exact_bitwise():
+    rng = np.random.default_rng(101)
+    for trial in range(30):
+        n = int(rng.integers(1, 13))
+        gates, customs = H.random_gates(n, 40, rng, with_custom=True)
+        say("exact trial", trial, "n", n)
+        p = H.to_program(gates, customs)
+        got = run_gpu(n, p, fused=0)
+        want = H.oracle_simulate(n, p)
+        assert np.array_equal(got, want), trial
+
+
+def test_random_circuits_fused():
+    rng = np.random.default_rng(202)
+    for trial in range(40):
+        n = int(rng.integers(1, 16))
+        gates, customs = H.random_gates(n, 60, rng, with_custom=(trial % 2 == 0))
+        say("fused trial", trial, "n", n)
+        p = H.to_program(gates, customs)
+        got = run_gpu(n, p, fused=1)
+        want = H.oracle_simulate(n, p)
+        assert np.max(np.abs(got - want)) < TOL_FUSED, trial
+
+
+@pytest.mark.parametrize("tile_bits", [2, 3, 4, 5, 6, 8, 10, 11, 12])
+def test_tile_widths(tile_bits):
+    rng = np.random.default_rng(300 + tile_bits)
+    n = 14
+    gates, _ = H.random_gates(n, 120, rng)
+    say("tile width", tile_bits)
+    p = H.to_program(gates)
+    got = run_gpu(n, p, fused=1, tile_bits=tile_bits)
+    want = H.oracle_simulate(n, p)
+    assert np.max(np.abs(got - want)) < TOL_FUSED
+
+
+def test_complex64_random():
+    rng = np.random.default_rng(404)
+    for trial in range(10):
+        n = int(rng.integers(2, 15))
+        gates, _ = H.random_gates(n, 50, rng)
+        say("c64 trial", trial, "n", n)
+        p = H.to_program(gates)
+        got = run_gpu(n, p, fused=1, precision=_capi.QR_C64)
+        want = H.oracle_simulate(n, p)
+        assert np.max(np.abs(got - want)) < TOL_C64, trial
+
+
+def test_qft10_state_and_bins():
+    """QFT on 10 qubits: full state and measure_all(500) bins against the oracle, with the
+    same 500 injected draws."""
+    for seed in range(5):
+        gates, _ = W.qft10(10, seed)
+        p = W.to_program(gates)
+        u = np.array(W.uniforms(500, seed + 1000))
+        want = H.oracle_simulate(10, p)
+        want_bins = H.oracle_measure(10, want, u)
+        for fused in (0, 1):
+            say("qft10 seed", seed, "fused", fused)
+            st = _capi.State(10)
+            st.set_options(fused=fused)
+            st.apply(p)
+            got = st.amplitudes()
+            if fused == 0:
+                assert np.array_equal(got, want)
+            assert np.max(np.abs(got - want)) < TOL_FUSED
+            bins = st.sample(u)
+            st.close()
+            if fused == 0:
+                assert np.array_equal(bins, want_bins)
+            else:  # fused state differs in the last bits: bins equal unless a draw sits on a boundary
+                assert np.mean(bins == want_bins) > 0.99
+
+
+def test_sampler_exact_fold_bitwise():
+    rng = np.random.default_rng(55)
+    for n in (1, 4, 9, 13, 16):
+        psi = rng.standard_normal(1 << n) + 1j * rng.standard_normal(1 << n)
+        psi /= np.linalg.norm(psi)
+        psi[rng.integers(1 << n)] = 0.0
+        u = rng.random(1000)
+        u[:4] = [0.0, 0.5, 1.0, 2.0]
+        say("sampler n", n)
+        st = _capi.State(n)
+        st.set_amplitudes(psi)
+        st.set_options(exact_sample=1)
+        got = st.sample(u)
+        want = H.oracle_measure(n, st.amplitudes(), u)
+        assert np.array_equal(got, want)
+        st.set_options(exact_sample=0)
+        got2 = st.sample(u)
+        assert np.mean(got2 == want) > 0.999
+        st.close()
+
+
+def test_sampler_none_for_non_unitary():
+    images = np.array([[np.sqrt(2.0), 0.0], [0.0, 0.0]], dtype=np.complex128)
+    p = _capi.Program()
+    for w in range(3):
+        p.add("H", w)
+    p.add_custom(1, [], images, [1, 1])
+    st = _capi.State(3)
+    st.apply(p)
+    out = st.sample([0.1, 0.99999, 0.999999999])
+    assert list(out[:2]) == [0, 5] and out[2] == 5
+    st.set_amplitudes(np.full(8, 0.25 + 0j))  # total probability 0.5
+    out = st.sample([0.2, 0.6])
+    assert out[0] == 3 and out[1] == np.iinfo(np.uint64).max
+    st.close()
+
+
+def test_brickwork_20q_vs_oracle():
+    n = 20
+    gates = W.brickwork(n, depth=40, seed=1234)
+    p = W.to_program(gates)
+    say("brickwork 20q oracle")
+    want = H.oracle_simulate(n, p, threads=8)
+    say("brickwork 20q gpu")
+    st = _capi.State(n)
+    st.apply(p)
+    got = st.amplitudes()
+    plan = st.last_plan()
+    st.close()
+    assert np.max(np.abs(got - want)) < TOL_FUSED
+    assert plan["sweeps"] < 120
+
+
+def test_norm_and_expectation():
+    rng = np.random.default_rng(77)
+    n = 12
+    gates, _ = H.random_gates(n, 80, rng)
+    p = H.to_program(gates)
+    st = _capi.State(n)
+    st.apply(p)
+    psi = st.amplitudes()
+    assert abs(st.norm_sqr() - 1.0) < 1e-12
+    obs = _capi.Observable(W.hea_observable(n) + [(0.3, [(2, "Y"), (5, "Y")]), (-0.2, [(1, "X"), (7, "Y"), (9, "Z")])])
+    got = st.expectation(obs)
+    want = H.oracle_expectation(n, psi, obs)
+    assert abs(got - want) < 1e-11
+    st.close()
+
+
+def _energy_oracle(n, gates, obs):
+    psi = H.oracle_simulate(n, W.to_program(gates))
+    return H.oracle_expectation(n, psi, obs)
+
+
+@pytest.mark.parametrize("precision,tol", [(_capi.QR_C128, 1e-9), (_capi.QR_C64, 1e-4)])
+def test_adjoint_vs_parameter_shift(precision, tol):
+    n, layers = 6, 3
+    gates, thetas = W.hea(n, layers, seed=5)
+    gates = gates + [("Phase", 2, (), 0.37, 0), ("Rx", 4, (), 1.1, 0), ("H", 1, (), 0.0, 0),
+                     ("Swap", 3, (0,), 0.0, 0), ("Rx", 0, (), -0.4, 0)]
+    obs = _capi.Observable(W.hea_observable(n))
+    g = _capi.Grad(n, precision)
+    e = g.forward(W.to_program(gates), obs)
+    grads = g.backward(1.0)
+    e_ref = _energy_oracle(n, gates, obs)
+    assert abs(e - e_ref) < (1e-12 if precision == _capi.QR_C128 else 1e-5)
+    pidx = [i for i, x in enumerate(gates) if x[0] in ("Rx", "Ry", "Rz", "Phase")]
+    assert len(grads) == len(pidx)
+    for k, i in enumerate(pidx):
+        kind, pos, ctl, arg, argi = gates[i]
+        if kind == "Phase":
+            ps = 0.0
+        else:
+            gp = list(gates); gp[i] = (kind, pos, ctl, arg + np.pi / 2, argi)
+            gm = list(gates); gm[i] = (kind, pos, ctl, arg - np.pi / 2, argi)
+            ps = 0.5 * (_energy_oracle(n, gp, obs) - _energy_oracle(n, gm, obs))
+        h = 1e-6
+        gp = list(gates); gp[i] = (kind, pos, ctl, arg + h, argi)
+        gm = list(gates); gm[i] = (kind, pos, ctl, arg - h, argi)
+        fd = (_energy_oracle(n, gp, obs) - _energy_oracle(n, gm, obs)) / (2 * h)
+        assert abs(grads[k] - ps) < tol, (k, kind)
+        assert abs(ps - fd) < 1e-7
+    g.close()
+
+
+def test_oom_is_reported():
+    h = _capi.State.__new__(_capi.State)
+    import ctypes as C
+    hh = C.c_void_p()
+    st = _capi.lib().qr_state_create(37, _capi.QR_C128, C.byref(hh))
+    assert st == _capi.QR_ERR_OOM
+    assert "allocate" in _capi.lib().qr_last_error().decode()
+
+
+def test_register_dimension_error():
+    st = _capi.State(3)
+    with pytest.raises(_capi.QrError) as e:
+        st.set_amplitudes(np.zeros(4, dtype=np.complex128))
+    assert str(e.value) == ("The custom register has a product state dimension of 2, while the number of "
+                            "qubits is 3. These must equal each other.")
+    st.close()
