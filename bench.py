@@ -1,0 +1,231 @@
+"""Benchmark of the quantr hot path on one MI355X.
+
+Workload (one "step"): the 30-qubit complex128 random brickwork circuit of depth 40 (1,788
+gates, seed 1234) simulated from |0...0> through the C ABI — state reset plus every gate.
+Inputs live in HBM before the timed region; the state never leaves the GPU.
+
+Prints ONE JSON line: value = algorithmic HBM GB/s of the whole step (bytes every sweep must
+read and write, plus the |0...0> write, divided by the step time); gates/s, the adjoint vs
+parameter-shift gradient rate (20 qubits, 400 parameters), the tile-sweep roofline and a CPU
+baseline (the oracle's OpenMP state vector on this host) ride along.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from quantr_amd import _capi  # noqa: E402
+from quantr_amd import workloads as W  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=30)
+    ap.add_argument("--depth", type=int, default=40)
+    ap.add_argument("--precision", choices=["c128", "c64"], default="c128")
+    ap.add_argument("--tile-bits", type=int, default=0)
+    ap.add_argument("--low-bits", type=int, default=0)
+    ap.add_argument("--tile-variant", type=int, default=0)
+    ap.add_argument("--reg-bits", type=int, default=0)
+    ap.add_argument("--jit", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-qubits", type=int, default=20)
+    ap.add_argument("--adjoint-layers", type=int, default=10)
+    ap.add_argument("--shift-evals", type=int, default=0,
+                    help="parameter-shift gradients timed (0 = all parameters)")
+    return ap.parse_args()
+
+
+def traffic_from_profile(n, prec):
+    """HBM bytes per tile sweep measured by rocprofv3 PMC runs (FETCH_SIZE + WRITE_SIZE, scaled
+    by the same counters on kernels of known traffic), if a summary was committed."""
+    p = os.path.join(ROOT, "profiles", "tile_sweep_%dq_%s_traffic.json" % (n, prec))
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return float(json.load(f)["traffic_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def bench_simulate(args):
+    n = args.qubits
+    prec = _capi.QR_C128 if args.precision == "c128" else _capi.QR_C64
+    B = 16 if prec == _capi.QR_C128 else 8
+    gates = W.brickwork(n, depth=args.depth, seed=1234)
+    prog = W.to_program(gates).c_struct()
+    st = _capi.State(n, prec)
+    st.set_options(fused=1, tile_bits=args.tile_bits, low_bits=args.low_bits, tile_variant=args.tile_variant,
+                   reg_bits=args.reg_bits, jit=args.jit)
+    log("bench: %d-qubit %s brickwork depth %d, %d gates" % (n, args.precision, args.depth, len(gates)))
+    for i in range(args.warmup):
+        st.reset()
+        st.apply(prog)
+        st.synchronize()
+        log("bench: warmup step %d done" % i)
+    plan = st.last_plan()
+    st.profile_enable(True)
+    st.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st.reset()
+        st.apply(prog)
+    st.synchronize()
+    t1 = time.perf_counter()
+    prof = st.profile_read()
+    st.profile_enable(False)
+    norm = st.norm_sqr()
+    st.close()
+    ms = (t1 - t0) * 1000.0 / args.steps
+    init_bytes = float(1 << n) * B
+    step_bytes = plan["bytes"] + init_bytes
+    log("bench: %.3f ms/step, %d sweeps (%d tile), norm-1 = %.2e" % (ms, plan["sweeps"], plan["tile_sweeps"], norm - 1))
+    return {"ms": ms, "plan": plan, "prof": prof, "step_bytes": step_bytes, "gates": len(gates), "n": n,
+            "B": B, "norm_err": abs(norm - 1.0)}
+
+
+def bench_adjoint(args):
+    """Adjoint gradient vs the plain GPU parameter-shift path on the HEA (20 qubits, 10
+    layers, 400 parameters), observable sum Z_i Z_i+1 + 0.5 sum X_i."""
+    n, L = args.adjoint_qubits, args.adjoint_layers
+    gates, thetas = W.hea(n, L, seed=2020)
+    obs = _capi.Observable(W.hea_observable(n))
+    P = len(thetas)
+    g = _capi.Grad(n, _capi.QR_C128)
+    e = g.forward(W.to_program(gates), obs)
+    g.backward()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.forward(W.to_program(gates), obs)
+        grads = g.backward()
+    t_adj = (time.perf_counter() - t0) / reps
+    g.close()
+    # parameter shift: two plain simulations + expectation per parameter
+    st = _capi.State(n, _capi.QR_C128)
+    pidx = [i for i, x in enumerate(gates) if x[0] in ("Rx", "Ry", "Rz")]
+    m = args.shift_evals or len(pidx)
+    shift = np.zeros(m)
+    t0 = time.perf_counter()
+    for j in range(m):
+        i = pidx[j]
+        vals = []
+        for s in (+1, -1):
+            kind, pos, ctl, arg, argi = gates[i]
+            gg = list(gates)
+            gg[i] = (kind, pos, ctl, arg + s * math.pi / 2, argi)
+            st.reset()
+            st.apply(W.to_program(gg))
+            vals.append(st.expectation(obs))
+        shift[j] = 0.5 * (vals[0] - vals[1])
+    t_ps = (time.perf_counter() - t0) / m * P
+    st.close()
+    err = float(np.max(np.abs(grads[:m] - shift)) / max(1e-300, float(np.max(np.abs(shift)))))
+    log("bench: adjoint %.2f ms for %d grads; parameter shift %.1f ms; rel err %.2e" % (t_adj * 1e3, P, t_ps * 1e3, err))
+    return {"adjoint_grads_per_s": P / t_adj, "param_shift_grads_per_s": P / t_ps,
+            "adjoint_vs_param_shift": t_ps / t_adj, "adjoint_rel_err_vs_shift": err, "energy": e,
+            "params": P, "qubits": n, "layers": L}
+
+
+def cpu_baseline(n, depth):
+    """The oracle's in-place state vector (C++/OpenMP, all host cores) on a bounded sample:
+    the first gates of the same brickwork at the same size, timed per gate."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers as H  # oracle binding (checker / baseline only)
+
+    threads = os.cpu_count() or 1
+    try:
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+    except ValueError:
+        pass
+    gates = W.brickwork(n, depth=depth, seed=1234)
+    state_gb = (1 << n) * 16 / 1e9
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1.0
+    # first gate touches every page of the state; it is not timed
+    H.oracle_apply_inplace(n, psi, W.to_program(gates[:1]), threads)
+    t0 = time.perf_counter()
+    H.oracle_apply_inplace(n, psi, W.to_program(gates[1:2]), threads)
+    t_gate = time.perf_counter() - t0
+    # about 20 s of CPU work: one full sweep per gate
+    sample = max(1, min(len(gates) - 2, int(20.0 / max(t_gate, 1e-4))))
+    t0 = time.perf_counter()
+    H.oracle_apply_inplace(n, psi, W.to_program(gates[2:2 + sample]), threads)
+    t = time.perf_counter() - t0
+    del psi
+    rate = sample / t
+    return {"value": rate, "unit": "gates/s", "cores": threads, "kind": "port",
+            "sample": "gates 3..%d of the %d-qubit c128 depth-%d brickwork (%.1f GB state), oracle "
+                      "state vector (oracle/quantr_oracle.cpp, OpenMP)" % (2 + sample, n, depth, state_gb)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    if _capi.device_count() < 1:
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(3)
+    r = bench_simulate(args)
+    ms = r["ms"]
+    tile = r["prof"].get("tile_sweep", {"launches": 0, "total_ms": 0.0, "bytes": 0.0})
+    avg_ms = tile["total_ms"] / max(tile["launches"], 1)
+    per_launch_bytes = tile["bytes"] / max(tile["launches"], 1)
+    achieved = per_launch_bytes / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = traffic_from_profile(r["n"], args.precision)
+    out = {
+        "metric": "HBM GB/s + gates/s at 30q c128; adjoint vs param-shift grads/s (20q,400p)",
+        "value": round(r["step_bytes"] / (ms * 1e-3) / 1e9, 2),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "c128" else "f32",
+        "data": "synthetic (seeded random brickwork circuit, SplitMix64 seed 1234)",
+        "config": {"workload": "random brickwork, %d qubits, %s, depth %d (%d gates), from |0...0>"
+                               % (r["n"], args.precision, args.depth, r["gates"]),
+                   "qubits": r["n"], "depth": args.depth, "gates": r["gates"], "precision": args.precision},
+        "gates_per_s": round(r["gates"] / (ms * 1e-3), 1),
+        "sweeps_per_step": r["plan"]["sweeps"],
+        "tile_sweeps_per_step": r["plan"]["tile_sweeps"],
+        "algorithmic_bytes_per_step": r["step_bytes"],
+        "norm_error": r["norm_err"],
+        "roofline": {"bound": "hbm", "kernel": "k_tile (fused tile sweep)",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch_bytes},
+    }
+    if not args.no_adjoint:
+        out["adjoint"] = bench_adjoint(args)
+    if not args.no_cpu_baseline:
+        log("bench: CPU baseline")
+        out["cpu_baseline"] = cpu_baseline(r["n"], args.depth)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,7 +132,8 @@ typedef struct qr_observable {
   uint64_t nfactors;
 } qr_observable;
 
-/* Execution options of a state (defaults: fused = 1, tile_bits = 0, low_bits = 0, exact_sample = 2). */
+/* Execution options of a state (defaults: fused = 1, tile_bits = 0, low_bits = 0, exact_sample = 2,
+ * tile_variant = 0, reg_bits = 0, jit = 2). */
 typedef struct qr_options {
   uint32_t fused;        /* 1: fused tile sweeps (FMA, <=1e-12 vs the reference order);
                             0: one sweep per gate in the reference's exact arithmetic order */
@@ -141,6 +142,12 @@ typedef struct qr_options {
   uint32_t exact_sample; /* 1: sampler reproduces the reference's sequential left fold
                             bit for bit; 0: parallel block scan (block sums reassociate
                             the fold); 2 (default): exact when n <= 26, else parallel   */
+  uint32_t tile_variant; /* 0: auto; 1: one tile in registers per workgroup;
+                            2: next tile prefetched into registers (double buffer)       */
+  uint32_t reg_bits;     /* amplitudes per thread = 2^reg_bits (0 = auto, 4 or 5)       */
+  uint32_t jit;          /* tile sweeps compiled for the circuit at run time (hipRTC):
+                            0 never; 1 always; 2 (default) from the second application of
+                            the same circuit structure on this state, or at once for n >= 28 */
 } qr_options;
 
 typedef struct qr_state qr_state;

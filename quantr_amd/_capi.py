@@ -43,7 +43,8 @@ class QrObservable(C.Structure):
 
 class QrOptions(C.Structure):
     _fields_ = [("fused", C.c_uint32), ("tile_bits", C.c_uint32), ("low_bits", C.c_uint32),
-                ("exact_sample", C.c_uint32)]
+                ("exact_sample", C.c_uint32), ("tile_variant", C.c_uint32), ("reg_bits", C.c_uint32),
+                ("jit", C.c_uint32)]
 
 
 class QrPlanStats(C.Structure):
@@ -242,8 +243,8 @@ class State:
     def handle(self):
         return self._h
 
-    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2):
-        o = QrOptions(fused, tile_bits, low_bits, exact_sample)
+    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit)
         check(lib().qr_state_set_options(self._h, C.byref(o)))
 
     def reset(self):

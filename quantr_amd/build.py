@@ -24,7 +24,20 @@ HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["kernels.hip", "lower.cpp", "planner.cpp", "capi.cpp"]
+SOURCES = ["kernels.hip", "lower.cpp", "planner.cpp", "capi.cpp", "jit.cpp"]
+
+
+def embed_tile_ops():
+    """tile_ops.hpp as a C++ string, compiled into the library for runtime specialisation."""
+    src = open(os.path.join(CSRC, "tile_ops.hpp")).read()
+    assert ")QRSRC\"" not in src
+    out = os.path.join(BUILD, "tile_ops_src.cpp")
+    text = ("namespace qr {\nextern const char* const kTileOpsSource;\n"
+            "const char* const kTileOpsSource = R\"QRSRC(" + src + ")QRSRC\";\n}\n")
+    if not os.path.exists(out) or open(out).read() != text:
+        with open(out, "w") as f:
+            f.write(text)
+    return out
 
 
 def _run(cmd):
@@ -40,10 +53,12 @@ def build_engine(verbose=False):
     os.makedirs(LIB, exist_ok=True)
     objs = []
     cmds = []
-    for src in SOURCES:
-        obj = os.path.join(BUILD, src + ".o")
+    embedded = embed_tile_ops()
+    for src in SOURCES + [embedded]:
+        path = src if os.path.isabs(src) else os.path.join(CSRC, src)
+        obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj])
         objs.append(obj)
     with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
         for out in ex.map(_run, cmds):
@@ -52,7 +67,7 @@ def build_engine(verbose=False):
     so = os.path.join(LIB, "libquantr_hip.so")
     tmp = so + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs +
-         ["-Wl,-soname,libquantr_hip.so"])
+         ["-lhiprtc", "-Wl,-soname,libquantr_hip.so"])
     os.replace(tmp, so)
     return so
 

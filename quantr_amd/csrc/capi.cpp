@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "engine.hpp"
+#include "jit.hpp"
 #include "kernels.hpp"
 
 namespace qr {
@@ -86,7 +87,8 @@ struct qr_state {
   void* amp = nullptr;
   hipStream_t stream = nullptr;
   int device = 0;
-  qr_options opt{1, 0, 0, 2};
+  qr_options opt{1, 0, 0, 2, 0, 0, 2};
+  std::vector<size_t> seen_structures;  // hashes of stage-structure lists run by the interpreter
   DevBuf aux;   // plan / coefficients / small uploads
   DevBuf aux2;  // partial sums, sampler scratch
   DevBuf tmp;   // wide-gate staging (bounded, never state-sized)
@@ -149,9 +151,11 @@ T* upload(qr_state* st, DevBuf& buf, const std::vector<T>& v) {
   return (T*)buf.p;
 }
 
+// Default tile shape: 2^12 complex128 / 2^13 complex64 amplitudes (64 KB of LDS, two
+// workgroups per CU), 16 amplitudes per thread.
 int pick_tile_bits(const qr_state* st) {
-  int k = tile_max_bits(st->prec);
-  if (st->opt.tile_bits) k = std::min<int>(k, (int)st->opt.tile_bits);
+  int k = st->prec == 0 ? 12 : 13;
+  if (st->opt.tile_bits) k = std::min<int>(tile_max_bits(st->prec), (int)st->opt.tile_bits);
   return std::min<int>(k, (int)st->n);
 }
 
@@ -159,7 +163,8 @@ PlanParams plan_params(const qr_state* st) {
   PlanParams pp;
   pp.tile_bits = pick_tile_bits(st);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
-  pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 4 : 5);
+  if (st->opt.reg_bits == 5 && pp.tile_bits >= 10) pp.reg_bits = 5;
+  pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
   return pp;
 }
 
@@ -168,8 +173,9 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
   const uint32_t n = st->n;
   const int B = (int)st->bytes_per_amp();
   std::vector<Stage> stages;
+  const PlanParams pp = plan_params(st);
   if (fused) {
-    stages = plan_fused(n, ops, plan_params(st), B, false);
+    stages = plan_fused(n, ops, pp, B, false);
   } else {
     for (size_t i = 0; i < ops.size(); ++i) {
       Stage s;
@@ -200,17 +206,30 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
     if (s.tile) {
       align(64);
       std::vector<TileOpDev> d(s.ops.size());
+      std::vector<double2> c(s.ops.size() * 4);
       for (size_t i = 0; i < s.ops.size(); ++i) {
         const TileOpHost& h = s.ops[i];
-        d[i] = TileOpDev{h.code, h.a, h.cmask, h.coef, h.b, h.c, h.d};
+        TileOpDev t{};
+        const bool maps_op = h.code == TC_REMAP || h.code == TC_SWAPSLOTS;
+        t.w0 = (h.code & 0xFF) | ((maps_op ? 0u : (h.a & 0xFF)) << 8) | ((h.b & 0xFF) << 16) | ((h.flags & 0xFF) << 24);
+        t.ns = h.ns;
+        t.lane_cmask = h.lane_cmask;
+        t.map = maps_op ? h.a : 0;
+        t.swz = h.swz;
+        d[i] = t;
+        for (int q = 0; q < 4; ++q) c[4 * i + q] = make_double2(h.coef[q].real(), h.coef[q].imag());
       }
       loc[si].ops = append(d.data(), d.size() * sizeof(TileOpDev));
       align(16);
-      std::vector<double2> c(s.coefs.size());
-      for (size_t i = 0; i < c.size(); ++i) c[i] = make_double2(s.coefs[i].real(), s.coefs[i].imag());
       loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
-      std::vector<uint8_t> m(s.maps.begin(), s.maps.end());
-      loc[si].maps = append(m.data(), m.size());
+      align(4);
+      std::vector<uint32_t> m(s.maps.size());
+      for (size_t i = 0; i < m.size(); ++i)
+        m[i] = (uint32_t)s.tbits[s.maps[i]] | ((uint32_t)s.maps[i] << 8);
+      loc[si].maps = append(m.data(), m.size() * sizeof(uint32_t));
+      std::vector<uint32_t> z(s.swz.begin(), s.swz.end());
+      if (z.empty()) z.push_back(0);
+      loc[si].off = append(z.data(), z.size() * sizeof(uint32_t));
     } else {
       const FastOp& op = ops[s.standalone];
       if (op.kind == FK_DENSEK || op.kind == FK_DIAGK) {
@@ -229,6 +248,24 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
     }
   }
   uint8_t* dblob = upload(st, st->aux, blob);
+  // ---- runtime-specialised stages --------------------------------------------------------
+  std::vector<std::string> jit_src;
+  bool use_jit = false;
+  if (fused && st->opt.jit != 0 && st->opt.tile_variant != 2) {
+    jit_src = jit_stage_sources(stages, st->prec, pp.reg_bits);
+    size_t h = 0;
+    for (const auto& x : jit_src) h = h * 1000003u ^ std::hash<std::string>()(x);
+    if (st->opt.jit == 1 || st->n >= 28) {
+      use_jit = true;
+    } else {
+      use_jit = std::find(st->seen_structures.begin(), st->seen_structures.end(), h) != st->seen_structures.end();
+      if (!use_jit) {
+        st->seen_structures.push_back(h);
+        if (st->seen_structures.size() > 64) st->seen_structures.erase(st->seen_structures.begin());
+      }
+    }
+    if (use_jit) jit_prepare(jit_src);
+  }
   // ---- launch --------------------------------------------------------------------------
   qr_plan_stats ps{};
   for (size_t si = 0; si < stages.size(); ++si) {
@@ -239,18 +276,21 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
       ps.tile_sweeps++;
       TileArgs a{};
       a.ops = (const TileOpDev*)(dblob + loc[si].ops);
-      a.maps = dblob + loc[si].maps;
       a.coefs = (const double2*)(dblob + loc[si].coefs);
+      a.maps = (const uint32_t*)(dblob + loc[si].maps);
+      a.swz = (const uint32_t*)(dblob + loc[si].off);
       a.nops = (uint32_t)s.ops.size();
-      for (int p = 0; p < s.k; ++p) a.tbits[p] = (uint8_t)s.tbits[p];
+      a.ntiles = 1ull << (n - s.k);
       uint64_t tm = 0;
       for (int b : s.tbits) tm |= 1ull << b;
-      int nn = 0;
-      for (uint32_t b = 0; b < n; ++b)
-        if (!(tm & (1ull << b))) a.nontile[nn++] = (uint8_t)b;
-      a.nnontile = nn;
+      a.nontile_mask = (n >= 64 ? ~0ull : ((1ull << n) - 1)) & ~tm;
       ProfScope ps_(st, "tile_sweep", s.bytes);
-      hip_check(launch_tile(st->prec, s.k, st->amp, 1ull << (n - s.k), a, st->stream), "tile kernel");
+      if (use_jit)
+        hip_check(jit_launch(jit_src[si], s.k, pp.reg_bits, st->amp, a.coefs, a.ntiles, a.nontile_mask, st->stream),
+                  "compiled tile kernel");
+      else
+        hip_check(launch_tile(st->prec, s.k, pp.reg_bits, st->opt.tile_variant == 2, st->amp, a, st->stream),
+                  "tile kernel");
       continue;
     }
     const FastOp& op = ops[s.standalone];
@@ -665,6 +705,9 @@ qr_status qr_state_set_options(qr_state* st, const qr_options* o) {
     if (!st || !o) throw Error{QR_ERR_ARG, "NULL argument"};
     if (o->tile_bits > 16) throw Error{QR_ERR_ARG, "tile_bits must be <= 16"};
     if (o->low_bits > 8) throw Error{QR_ERR_ARG, "low_bits must be <= 8"};
+    if (o->tile_variant > 2) throw Error{QR_ERR_ARG, "tile_variant must be 0, 1 or 2"};
+    if (o->reg_bits != 0 && o->reg_bits != 4 && o->reg_bits != 5) throw Error{QR_ERR_ARG, "reg_bits must be 0, 4 or 5"};
+    if (o->jit > 2) throw Error{QR_ERR_ARG, "jit must be 0, 1 or 2"};
     st->opt = *o;
   });
 }

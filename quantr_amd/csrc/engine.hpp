@@ -84,13 +84,14 @@ FastOp adjoint(const FastOp& op);
 // ---- planning -------------------------------------------------------------------------------
 
 struct TileOpHost {
-  uint32_t code;
-  uint32_t a;
-  uint64_t cmask;
-  uint32_t coef;   // index into the coefficient array (complex units)
-  uint32_t b;
-  uint32_t c;
-  uint32_t d;
+  uint32_t code = 0;
+  uint32_t a = 0;        // U1: register slot; D1: global bit; REMAP/SWAPSLOTS: new map index
+  uint32_t b = 0;        // U1: form; D1: register slot holding the bit, or 0xFF
+  uint32_t flags = 0;    // D1: bit 0 = only the |1> half changes; bits 4-7 = factor form
+  uint32_t ns = 0;       // register slots that must be 1 (controls held in registers)
+  uint64_t lane_cmask = 0;  // control bits held by thread slots or outside the tile
+  uint32_t swz = 0;      // REMAP: index of the LDS swizzle table
+  cd coef[4];
 };
 
 struct Stage {
@@ -98,8 +99,8 @@ struct Stage {
   int k = 0;                   // tile bits
   std::vector<int> tbits;      // tile bits ascending (global positions)
   std::vector<int> maps;       // flat: maps[i*k + s] = tile position held by slot s
+  std::vector<uint16_t> swz;   // flat, per remap: LDS pattern of each slot, old map then new map
   std::vector<TileOpHost> ops;
-  std::vector<cd> coefs;
   int standalone = -1;         // index into the op list for standalone stages
   double bytes = 0.0;          // algorithmic bytes of this sweep (2 * 2^n * B * f_touch)
   int ngates = 0;

@@ -1,0 +1,263 @@
+// jit.cpp — runtime-specialised tile sweeps.
+//
+// The interpreted tile sweep (k_tile in kernels.hip) decodes every op at run time; its
+// dispatch switch costs the compiler register copies and branches on every op. For circuits
+// that are applied repeatedly (benchmarks, gradient loops, serving) each stage's op list is
+// instead turned into straight-line code: the same register-level gate templates
+// (tile_ops.hpp, compiled from its text with hipRTC), with register slots, forms, control
+// masks, slot maps and LDS patterns as compile-time constants. Coefficients stay run-time
+// data, so a compiled stage serves every circuit of the same structure (e.g. all
+// parameter-shifted variants). Modules are cached per process, keyed by their source text.
+#include <hip/hip_runtime.h>
+#include <hip/hiprtc.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <sstream>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "engine.hpp"
+#include "jit.hpp"
+#include "kernels.hpp"
+
+namespace qr {
+
+extern const char* const kTileOpsSource;  // text of tile_ops.hpp, embedded at build time
+
+namespace {
+
+struct JitKernel {
+  hipModule_t mod = nullptr;
+  hipFunction_t fn = nullptr;
+};
+
+std::mutex g_mu;
+std::map<std::string, JitKernel> g_cache;
+
+std::string hex64(uint64_t v) {
+  char b[32];
+  std::snprintf(b, sizeof b, "0x%llxull", (unsigned long long)v);
+  return b;
+}
+
+// Source of one stage kernel. `prec` 0 = complex128, 1 = complex64.
+std::string stage_source(const Stage& s, int prec, int RB) {
+  const int K = s.k, NT = K - RB, NR = 1 << RB;
+  const char* R = prec == 0 ? "double" : "float";
+  const char* C = prec == 0 ? "double2" : "float2";
+  auto gbit = [&](int mi, int slot) { return s.tbits[s.maps[(size_t)mi * K + slot]]; };
+  auto thread_part = [&](int mi) {
+    std::ostringstream os;
+    os << "base";
+    for (int sl = 0; sl < NT; ++sl) os << " | ((uint64_t)((tid >> " << sl << ") & 1u) << " << gbit(mi, sl) << ")";
+    return os.str();
+  };
+  auto reg_off = [&](int mi, int m) {
+    uint64_t x = 0;
+    for (int j = 0; j < RB; ++j)
+      if ((m >> j) & 1) x |= 1ull << gbit(mi, NT + j);
+    return x;
+  };
+  std::ostringstream o;
+  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << NT) << ") void qr_stage(" << C
+    << "* __restrict__ psi, const double2* __restrict__ cf, uint64_t nontile) {\n";
+  o << "  using R = " << R << ";\n";
+  o << "  __shared__ " << C << " lds[" << (1 << K) << "];\n";
+  o << "  const uint32_t tid = threadIdx.x;\n";
+  o << "  uint64_t base = 0;\n";
+  o << "  { uint64_t m = nontile, t = blockIdx.x;\n"
+       "    while (m) { const uint64_t low = m & (0 - m); if (t & 1) base |= low; t >>= 1; m ^= low; } }\n";
+  o << "  qr::dev::Regs<R, " << RB << "> r;\n";
+  int mi = 0;
+  o << "  uint64_t g0 = " << thread_part(mi) << ";\n";
+  for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "] = psi[g0 | " << hex64(reg_off(mi, m)) << "];\n";
+  auto coef = [&](size_t i) {
+    std::ostringstream c;
+    c << "qr::dev::Coef4{{cf[" << 4 * i << "], cf[" << 4 * i + 1 << "], cf[" << 4 * i + 2 << "], cf["
+      << 4 * i + 3 << "]}}";
+    return c.str();
+  };
+  for (size_t i = 0; i < s.ops.size(); ++i) {
+    const TileOpHost& op = s.ops[i];
+    const std::string ns = std::to_string(op.ns) + "u";
+    const char* has_ns = op.ns ? "true" : "false";
+    switch (op.code) {
+      case TC_U1: {
+        if (op.lane_cmask == 0) {
+          o << "  qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(r, "
+            << coef(i) << ", " << ns << ");\n";
+        } else {
+          o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
+            << "    qr::dev::Coef4 u = " << coef(i) << ";\n"
+            << "    u.c[0] = ok ? u.c[0] : make_double2(1.0, 0.0); u.c[1] = ok ? u.c[1] : make_double2(0.0, 0.0);\n"
+            << "    u.c[2] = ok ? u.c[2] : make_double2(0.0, 0.0); u.c[3] = ok ? u.c[3] : make_double2(1.0, 0.0);\n"
+            << "    qr::dev::u1_slot<R, " << RB << ", " << op.a << ", 0, " << has_ns << ">(r, u, " << ns << "); }\n";
+        }
+        break;
+      }
+      case TC_D1: {
+        const bool only1 = op.flags & 1;
+        const int form = (int)(op.flags >> 4);
+        if (op.b != 0xFFu && op.lane_cmask == 0) {
+          o << "  qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", "
+            << (only1 ? form : 0) << ", " << has_ns << ">(r, " << coef(i) << ", " << ns << ");\n";
+        } else if (op.b != 0xFFu) {
+          o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
+            << "    qr::dev::Coef4 d = " << coef(i) << ";\n"
+            << "    d.c[0] = ok ? d.c[0] : make_double2(1.0, 0.0); d.c[1] = ok ? d.c[1] : make_double2(1.0, 0.0);\n"
+            << "    qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", 0, "
+            << has_ns << ">(r, d, " << ns << "); }\n";
+        } else {
+          o << "  { const bool bit = (g0 >> " << op.a << ") & 1;\n"
+            << "    double2 f = bit ? cf[" << 4 * i + 1 << "] : cf[" << 4 * i << "];\n";
+          if (op.lane_cmask)
+            o << "    const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
+              << "    f = ok ? f : make_double2(1.0, 0.0);\n";
+          o << "    qr::dev::scale_all<R, " << RB << ", 0, " << has_ns << ">(r, (R)f.x, (R)f.y, " << ns << "); }\n";
+        }
+        break;
+      }
+      case TC_REMAP: {
+        const uint16_t* P = &s.swz[(size_t)op.swz * 2 * K];
+        auto lds_thread = [&](const uint16_t* Q) {
+          std::ostringstream os;
+          os << "0u";
+          for (int sl = 0; sl < NT; ++sl) os << " ^ ((0u - ((tid >> " << sl << ") & 1u)) & " << Q[sl] << "u)";
+          return os.str();
+        };
+        auto lds_reg = [&](const uint16_t* Q, int m) {
+          uint32_t x = 0;
+          for (int j = 0; j < RB; ++j)
+            if ((m >> j) & 1) x ^= Q[NT + j];
+          return x;
+        };
+        o << "  { __syncthreads();\n    const uint32_t w = " << lds_thread(P) << ";\n";
+        for (int m = 0; m < NR; ++m) o << "    lds[w ^ " << lds_reg(P, m) << "u] = r.v[" << m << "];\n";
+        o << "    __syncthreads();\n    const uint32_t q = " << lds_thread(P + K) << ";\n";
+        for (int m = 0; m < NR; ++m) o << "    r.v[" << m << "] = lds[q ^ " << lds_reg(P + K, m) << "u];\n";
+        mi = (int)op.a;
+        o << "    g0 = " << thread_part(mi) << "; }\n";
+        break;
+      }
+      case TC_SWAPSLOTS:
+        mi = (int)op.a;
+        o << "  g0 = " << thread_part(mi) << ";\n";
+        break;
+      default:
+        break;
+    }
+  }
+  for (int m = 0; m < NR; ++m) o << "  psi[g0 | " << hex64(reg_off(mi, m)) << "] = r.v[" << m << "];\n";
+  o << "}\n";
+  return o.str();
+}
+
+std::string full_source(const std::string& kernel) {
+  std::string src = "typedef unsigned int uint32_t;\ntypedef unsigned long long uint64_t;\n";
+  src += kTileOpsSource;
+  src += "\n";
+  src += kernel;
+  return src;
+}
+
+// Compiles one program; returns the code object or an error message.
+bool compile_one(const std::string& src, std::vector<char>& code, std::string& err) {
+  hiprtcProgram prog;
+  if (hiprtcCreateProgram(&prog, src.c_str(), "qr_stage.hip", 0, nullptr, nullptr) != HIPRTC_SUCCESS) {
+    err = "hiprtcCreateProgram failed";
+    return false;
+  }
+  const char* opts[] = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                        "-Xclang", "-packed-fp32-ops"};
+  const hiprtcResult r = hiprtcCompileProgram(prog, 7, opts);
+  if (r != HIPRTC_SUCCESS) {
+    size_t ls = 0;
+    hiprtcGetProgramLogSize(prog, &ls);
+    std::string log(ls, '\0');
+    if (ls) hiprtcGetProgramLog(prog, &log[0]);
+    err = std::string("hiprtc: ") + hiprtcGetErrorString(r) + "\n" + log.substr(0, 2000);
+    hiprtcDestroyProgram(&prog);
+    return false;
+  }
+  size_t sz = 0;
+  hiprtcGetCodeSize(prog, &sz);
+  code.resize(sz);
+  hiprtcGetCode(prog, code.data());
+  hiprtcDestroyProgram(&prog);
+  return true;
+}
+
+}  // namespace
+
+std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB) {
+  std::vector<std::string> out(stages.size());
+  for (size_t i = 0; i < stages.size(); ++i)
+    if (stages[i].tile) out[i] = stage_source(stages[i], prec, RB);
+  return out;
+}
+
+void jit_prepare(const std::vector<std::string>& sources) {
+  std::vector<std::string> todo;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    for (const auto& s : sources)
+      if (!s.empty() && !g_cache.count(s) && std::find(todo.begin(), todo.end(), s) == todo.end()) todo.push_back(s);
+  }
+  if (todo.empty()) return;
+  std::vector<std::vector<char>> codes(todo.size());
+  std::vector<std::string> errs(todo.size());
+  std::vector<char> ok(todo.size(), 0);
+  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  size_t next = 0;
+  std::mutex qm;
+  auto worker = [&]() {
+    while (true) {
+      size_t i;
+      {
+        std::lock_guard<std::mutex> lk(qm);
+        if (next >= todo.size()) return;
+        i = next++;
+      }
+      ok[i] = compile_one(full_source(todo[i]), codes[i], errs[i]) ? 1 : 0;
+    }
+  };
+  std::vector<std::thread> th;
+  for (unsigned t = 0; t < std::min<size_t>(hw, todo.size()); ++t) th.emplace_back(worker);
+  for (auto& t : th) t.join();
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t i = 0; i < todo.size(); ++i) {
+    if (!ok[i]) throw Error{QR_ERR_HIP, "runtime compilation of a tile stage failed: " + errs[i]};
+    JitKernel k;
+    hipError_t e = hipModuleLoadData(&k.mod, codes[i].data());
+    if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e)};
+    e = hipModuleGetFunction(&k.fn, k.mod, "qr_stage");
+    if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleGetFunction: ") + hipGetErrorString(e)};
+    g_cache[todo[i]] = k;
+  }
+}
+
+hipError_t jit_launch(const std::string& source, int K, int RB, void* psi, const double2* coefs, uint64_t ntiles,
+                      uint64_t nontile_mask, hipStream_t s) {
+  hipFunction_t fn = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = g_cache.find(source);
+    if (it == g_cache.end()) return hipErrorNotFound;
+    fn = it->second.fn;
+  }
+  if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
+  void* args[] = {&psi, (void*)&coefs, &nontile_mask};
+  return hipModuleLaunchKernel(fn, (unsigned)ntiles, 1, 1, 1u << (K - RB), 1, 1, 0, s, args, nullptr);
+}
+
+size_t jit_cache_size() {
+  std::lock_guard<std::mutex> lk(g_mu);
+  return g_cache.size();
+}
+
+}  // namespace qr

@@ -5,23 +5,14 @@
 // loaded as one 16-byte dwordx4, complex64 as one 8-byte dwordx2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 
 #include "kernels.hpp"
+#include "tile_ops.hpp"
 
 namespace qr {
 namespace dev {
-
-template <typename R>
-struct CT;
-template <>
-struct CT<double> {
-  using T = double2;
-};
-template <>
-struct CT<float> {
-  using T = float2;
-};
 
 // Inserts a zero bit at each position of `sorted_bits` (ascending) into x.
 __device__ __forceinline__ uint64_t insert_zero_bits(uint64_t x, const uint8_t* sorted_bits, int nb) {
@@ -266,141 +257,21 @@ __global__ void k_diagk(typename CT<R>::T* __restrict__ psi, uint64_t dim, DiagK
 // Fused tile sweep (interpreter). See planner.cpp for the program it runs.
 // =====================================================================================
 
-template <typename R, int RB>
-struct Regs {
-  typename CT<R>::T v[1 << RB];
-};
-
-// Applies a 2x2 on register slot J to every register pair whose control predicate holds.
-template <typename R, int RB, int J, int FORM, bool CTRL>
-__device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const double2* __restrict__ u, uint32_t ns,
-                                        bool lane_ok) {
-  const R u00r = (R)u[0].x, u00i = (R)u[0].y, u01r = (R)u[1].x, u01i = (R)u[1].y;
-  const R u10r = (R)u[2].x, u10i = (R)u[2].y, u11r = (R)u[3].x, u11i = (R)u[3].y;
-#pragma unroll
-  for (int m0 = 0; m0 < (1 << RB); ++m0) {
-    if (m0 & (1 << J)) continue;
-    const int m1 = m0 | (1 << J);
-    if (CTRL) {
-      if (!lane_ok || ((uint32_t)m0 & ns) != ns) continue;
-    }
-    const R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
-    R o0r, o0i, o1r, o1i;
-    if (FORM == 3) {  // X: swap
-      o0r = yr; o0i = yi; o1r = xr; o1i = xi;
-    } else if (FORM == 1) {  // anti-diagonal
-      o0r = u01r * yr - u01i * yi;
-      o0i = u01r * yi + u01i * yr;
-      o1r = u10r * xr - u10i * xi;
-      o1i = u10r * xi + u10i * xr;
-    } else if (FORM == 2) {  // real
-      o0r = fma(u00r, xr, u01r * yr);
-      o0i = fma(u00r, xi, u01r * yi);
-      o1r = fma(u10r, xr, u11r * yr);
-      o1i = fma(u10r, xi, u11r * yi);
-    } else {
-      o0r = u00r * xr; o0i = u00r * xi;
-      o0r = fma(-u00i, xi, o0r); o0i = fma(u00i, xr, o0i);
-      o0r = fma(u01r, yr, o0r); o0i = fma(u01r, yi, o0i);
-      o0r = fma(-u01i, yi, o0r); o0i = fma(u01i, yr, o0i);
-      o1r = u10r * xr; o1i = u10r * xi;
-      o1r = fma(-u10i, xi, o1r); o1i = fma(u10i, xr, o1i);
-      o1r = fma(u11r, yr, o1r); o1i = fma(u11r, yi, o1i);
-      o1r = fma(-u11i, yi, o1r); o1i = fma(u11i, yr, o1i);
-    }
-    a.v[m0].x = o0r; a.v[m0].y = o0i;
-    a.v[m1].x = o1r; a.v[m1].y = o1i;
-  }
-}
-
-template <typename R, int RB, int J, bool CTRL>
-__device__ __forceinline__ void u1_dispatch_form(Regs<R, RB>& a, const double2* u, uint32_t form,
-                                                 uint32_t ns, bool lane_ok) {
-  switch (form) {
-    case 3: u1_slot<R, RB, J, 3, CTRL>(a, u, ns, lane_ok); break;
-    case 1: u1_slot<R, RB, J, 1, CTRL>(a, u, ns, lane_ok); break;
-    case 2: u1_slot<R, RB, J, 2, CTRL>(a, u, ns, lane_ok); break;
-    default: u1_slot<R, RB, J, 0, CTRL>(a, u, ns, lane_ok); break;
-  }
-}
-
-template <typename R, int RB, bool CTRL>
-__device__ __forceinline__ void u1_dispatch(Regs<R, RB>& a, uint32_t slot, const double2* u,
-                                            uint32_t form, uint32_t ns, bool lane_ok) {
-  switch (slot) {
-    case 0: u1_dispatch_form<R, RB, 0, CTRL>(a, u, form, ns, lane_ok); break;
-    case 1: if constexpr (RB > 1) u1_dispatch_form<R, RB, 1, CTRL>(a, u, form, ns, lane_ok); break;
-    case 2: if constexpr (RB > 2) u1_dispatch_form<R, RB, 2, CTRL>(a, u, form, ns, lane_ok); break;
-    case 3: if constexpr (RB > 3) u1_dispatch_form<R, RB, 3, CTRL>(a, u, form, ns, lane_ok); break;
-    case 4: if constexpr (RB > 4) u1_dispatch_form<R, RB, 4, CTRL>(a, u, form, ns, lane_ok); break;
-    default: break;
-  }
-}
-
-// diag(d0,d1) where the bit is register slot J (compile-time) or a per-thread bit (J < 0).
-template <typename R, int RB, int J, bool ONLY1>
-__device__ __forceinline__ void d1_apply(Regs<R, RB>& a, const double2* __restrict__ d, bool lane_bit,
-                                         uint32_t ns, bool lane_ok) {
-  const R d0r = (R)d[0].x, d0i = (R)d[0].y, d1r = (R)d[1].x, d1i = (R)d[1].y;
-#pragma unroll
-  for (int m = 0; m < (1 << RB); ++m) {
-    if (((uint32_t)m & ns) != ns) continue;
-    bool bit;
-    if (J >= 0) bit = (m >> (J >= 0 ? J : 0)) & 1;
-    else bit = lane_bit;
-    if (J >= 0 && ONLY1 && !bit) continue;
-    const bool act = lane_ok && (!ONLY1 || bit);
-    const R fr = bit ? d1r : d0r, fi = bit ? d1i : d0i;
-    const R xr = a.v[m].x, xi = a.v[m].y;
-    const R yr = fma(fr, xr, -fi * xi);
-    const R yi = fma(fr, xi, fi * xr);
-    a.v[m].x = act ? yr : xr;
-    a.v[m].y = act ? yi : xi;
-  }
-}
-
-template <typename R, int RB, bool ONLY1>
-__device__ __forceinline__ void d1_dispatch(Regs<R, RB>& a, uint32_t slot, const double2* d,
-                                            bool lane_bit, uint32_t ns, bool lane_ok) {
-  switch (slot) {
-    case 0: d1_apply<R, RB, 0, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-    case 1: if constexpr (RB > 1) d1_apply<R, RB, 1, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-    case 2: if constexpr (RB > 2) d1_apply<R, RB, 2, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-    case 3: if constexpr (RB > 3) d1_apply<R, RB, 3, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-    case 4: if constexpr (RB > 4) d1_apply<R, RB, 4, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-    default: d1_apply<R, RB, -1, ONLY1>(a, d, lane_bit, ns, lane_ok); break;
-  }
-}
-
-template <typename R, int K, int RB>
+template <int K, int RB>
 struct TileCtx {
   static constexpr int NT = K - RB;
-  uint64_t g0;        // global index of register 0 of this thread
-  uint32_t l0;        // tile-local index of register 0
-  uint64_t gb[RB];    // global bit of each register slot
-  uint32_t lb[RB];    // tile-local bit of each register slot
-  uint64_t regmask;   // OR of gb
-
-  __device__ __forceinline__ void setup(const uint8_t* __restrict__ mp, const TileArgs& a,
-                                        uint64_t base, uint32_t tid) {
+  uint64_t g0;       // global index of register 0 of this thread
+  uint64_t gb[RB];   // global bit of each register slot
+  // Branch-free: every map word is loaded (uniformly) before any per-lane use.
+  __device__ __forceinline__ void setup(const uint32_t* __restrict__ mp, uint64_t base, uint32_t tid) {
+    uint32_t w[K];
+#pragma unroll
+    for (int s = 0; s < K; ++s) w[s] = mp[s];
     g0 = base;
-    l0 = 0;
 #pragma unroll
-    for (int s = 0; s < NT; ++s) {
-      const int p = mp[s];
-      if ((tid >> s) & 1) {
-        g0 |= 1ull << a.tbits[p];
-        l0 |= 1u << p;
-      }
-    }
-    regmask = 0;
+    for (int s = 0; s < NT; ++s) g0 |= (uint64_t)((tid >> s) & 1u) << (w[s] & 0xFF);
 #pragma unroll
-    for (int j = 0; j < RB; ++j) {
-      const int p = mp[NT + j];
-      gb[j] = 1ull << a.tbits[p];
-      lb[j] = 1u << p;
-      regmask |= gb[j];
-    }
+    for (int j = 0; j < RB; ++j) gb[j] = 1ull << (w[NT + j] & 0xFF);
   }
   __device__ __forceinline__ uint64_t gidx(int m) const {
     uint64_t x = g0;
@@ -409,86 +280,167 @@ struct TileCtx {
       if ((m >> j) & 1) x |= gb[j];
     return x;
   }
-  __device__ __forceinline__ uint32_t lidx(int m) const {
-    uint32_t x = l0;
-#pragma unroll
-    for (int j = 0; j < RB; ++j)
-      if ((m >> j) & 1) x |= lb[j];
-    return x;
-  }
-  // Control predicate split: lane_ok = the thread/base part cannot fail; ns = register slots
-  // that must be 1.
-  __device__ __forceinline__ void controls(uint64_t cmask, uint32_t& ns, bool& lane_ok) const {
-    const uint64_t need = cmask & ~g0;
-    lane_ok = (need & ~regmask) == 0;
-    ns = 0;
-#pragma unroll
-    for (int j = 0; j < RB; ++j)
-      if (need & gb[j]) ns |= 1u << j;
-  }
 };
 
-template <typename R, int K, int RB>
+// LDS offsets of a thread's registers from per-slot patterns P (XOR-linear in the slots).
+template <int K, int RB>
+__device__ __forceinline__ void lds_offsets(const uint32_t* __restrict__ P, uint32_t tid, uint32_t& t0,
+                                            uint32_t (&rb)[RB]) {
+  constexpr int NT = K - RB;
+  uint32_t w[K];
+#pragma unroll
+  for (int s = 0; s < K; ++s) w[s] = P[s];
+  t0 = 0;
+#pragma unroll
+  for (int s = 0; s < NT; ++s) t0 ^= w[s] & (0u - ((tid >> s) & 1u));
+#pragma unroll
+  for (int j = 0; j < RB; ++j) rb[j] = w[NT + j];
+}
+
+template <int RB>
+__device__ __forceinline__ uint32_t lds_at(uint32_t t0, const uint32_t (&rb)[RB], int m) {
+  uint32_t x = t0;
+#pragma unroll
+  for (int j = 0; j < RB; ++j)
+    if ((m >> j) & 1) x ^= rb[j];
+  return x;
+}
+
+// An op and its four coefficients, loaded together (scalar loads).
+struct OpPack {
+  TileOpDev op;
+  Coef4 cf;
+};
+
+__device__ __forceinline__ void load_op(const TileArgs& a, uint32_t o, OpPack& p) {
+  p.op = a.ops[o];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) p.cf.c[q] = a.coefs[4 * (size_t)o + q];
+}
+
+// Deposits the bits of the tile number into the non-tile index bits (uniform scalar work).
+__device__ __forceinline__ uint64_t tile_base(uint64_t tile, const TileArgs& a) {
+  uint64_t base = 0, m = a.nontile_mask;
+  while (m) {
+    const uint64_t low = m & (0 - m);
+    if (tile & 1) base |= low;
+    tile >>= 1;
+    m ^= low;
+  }
+  return base;
+}
+
+// Persistent tile sweep: each workgroup walks tiles blockIdx.x, +gridDim.x, ...; the next
+// tile's amplitudes are loaded into registers while the current tile is processed.
+template <typename R, int K, int RB, bool PF>
 __global__ __launch_bounds__(1 << (K - RB)) void k_tile(typename CT<R>::T* __restrict__ psi, TileArgs a) {
   using C = typename CT<R>::T;
+  constexpr int NT = K - RB;
+  constexpr int NR = 1 << RB;
   __shared__ C lds[1 << K];
   const uint32_t tid = threadIdx.x;
-  uint64_t base = 0;
-  {
-    const uint64_t b = blockIdx.x;
-    for (int i = 0; i < a.nnontile; ++i) base |= ((b >> i) & 1ull) << a.nontile[i];
-  }
-  TileCtx<R, K, RB> ctx;
-  ctx.setup(a.maps, a, base, tid);
-  Regs<R, RB> r;
+  uint64_t tile = blockIdx.x;
+  if (tile >= a.ntiles) return;
+  TileCtx<K, RB> c0;  // load map, thread part only
+  c0.setup(a.maps, 0, tid);
+  uint64_t base = tile_base(tile, a);
+  Regs<R, RB> cur, nxt;
 #pragma unroll
-  for (int m = 0; m < (1 << RB); ++m) r.v[m] = psi[ctx.gidx(m)];
-
-  for (uint32_t o = 0; o < a.nops; ++o) {
-    const TileOpDev op = a.ops[o];
-    switch (op.code) {
-      case TC_U1: {
-        const double2* u = a.coefs + op.coef;
-        if (op.cmask == 0) {
-          u1_dispatch<R, RB, false>(r, op.a, u, op.b, 0, true);
-        } else {
-          uint32_t ns;
-          bool ok;
-          ctx.controls(op.cmask, ns, ok);
-          u1_dispatch<R, RB, true>(r, op.a, u, op.b, ns, ok);
+  for (int m = 0; m < NR; ++m) cur.v[m] = psi[base | c0.gidx(m)];
+  while (true) {
+    const uint64_t ntile = tile + gridDim.x;
+    const bool more = ntile < a.ntiles;
+    uint64_t nbase = 0;
+    if (PF && more) {
+      nbase = tile_base(ntile, a);
+#pragma unroll
+      for (int m = 0; m < NR; ++m) nxt.v[m] = psi[nbase | c0.gidx(m)];
+    }
+    TileCtx<K, RB> ctx;
+    ctx.g0 = base | c0.g0;
+#pragma unroll
+    for (int j = 0; j < RB; ++j) ctx.gb[j] = c0.gb[j];
+    for (uint32_t o = 0; o < a.nops; ++o) {
+      OpPack pk;
+      load_op(a, o, pk);
+      const TileOpDev& op = pk.op;
+      const uint32_t code = op.w0 & 0xFF, A = (op.w0 >> 8) & 0xFF, Bv = (op.w0 >> 16) & 0xFF, fl = op.w0 >> 24;
+      const bool lane_ok = (ctx.g0 & op.lane_cmask) == op.lane_cmask;  // used only if lane_cmask != 0
+      const Coef4 cf = pk.cf;
+      switch (code) {
+        // Only uniform branches below: controls held by threads become per-lane selects of
+        // the coefficients (a divergent branch would make the compiler copy every register).
+        case TC_U1:
+          if (op.lane_cmask == 0) {
+            if (op.ns) u1_dispatch<R, RB, true>(cur, A, cf, Bv, op.ns);
+            else u1_dispatch<R, RB, false>(cur, A, cf, Bv, 0);
+          } else {  // per-lane coefficients: the gate or the identity
+            Coef4 u;
+            u.c[0] = lane_ok ? cf.c[0] : make_double2(1.0, 0.0);
+            u.c[1] = lane_ok ? cf.c[1] : make_double2(0.0, 0.0);
+            u.c[2] = lane_ok ? cf.c[2] : make_double2(0.0, 0.0);
+            u.c[3] = lane_ok ? cf.c[3] : make_double2(1.0, 0.0);
+            if (op.ns) u1_dispatch<R, RB, true>(cur, A, u, 0, op.ns);
+            else u1_dispatch<R, RB, false>(cur, A, u, 0, 0);
+          }
+          break;
+        case TC_D1: {
+          if (Bv != 0xFF && op.lane_cmask == 0) {
+            if (op.ns) d1_reg_dispatch<R, RB, true>(cur, Bv, cf, fl, op.ns);
+            else d1_reg_dispatch<R, RB, false>(cur, Bv, cf, fl, 0);
+          } else if (Bv != 0xFF) {
+            Coef4 d;
+            d.c[0] = lane_ok ? cf.c[0] : make_double2(1.0, 0.0);
+            d.c[1] = lane_ok ? cf.c[1] : make_double2(1.0, 0.0);
+            d.c[2] = d.c[3] = make_double2(0.0, 0.0);
+            const uint32_t f2 = fl & 1;  // general factors, |1>-only skipping kept
+            if (op.ns) d1_reg_dispatch<R, RB, true>(cur, Bv, d, f2, op.ns);
+            else d1_reg_dispatch<R, RB, false>(cur, Bv, d, f2, 0);
+          } else {
+            const bool bit = (ctx.g0 >> A) & 1;
+            double2 f = bit ? cf.c[1] : cf.c[0];
+            if (op.lane_cmask) f = lane_ok ? f : make_double2(1.0, 0.0);
+            if (op.ns) scale_all<R, RB, 0, true>(cur, (R)f.x, (R)f.y, op.ns);
+            else scale_all<R, RB, 0, false>(cur, (R)f.x, (R)f.y, 0);
+          }
+          break;
         }
-        break;
-      }
-      case TC_D1: {
-        const double2* d = a.coefs + op.coef;
-        uint32_t ns = 0;
-        bool ok = true;
-        if (op.cmask) ctx.controls(op.cmask, ns, ok);
-        const bool lane_bit = (ctx.g0 >> op.a) & 1;
-        if (op.c) d1_dispatch<R, RB, true>(r, op.b, d, lane_bit, ns, ok);
-        else d1_dispatch<R, RB, false>(r, op.b, d, lane_bit, ns, ok);
-        break;
-      }
-      case TC_REMAP: {
-        __syncthreads();
+        case TC_REMAP: {
+          const uint32_t* P = a.swz + (size_t)(2 * K) * op.swz;
+          const uint32_t* nm = a.maps + (size_t)K * op.map;
+          uint32_t t0, rb[RB];
+          lds_offsets<K, RB>(P, tid, t0, rb);
+          __syncthreads();
 #pragma unroll
-        for (int m = 0; m < (1 << RB); ++m) lds[ctx.lidx(m)] = r.v[m];
-        __syncthreads();
-        ctx.setup(a.maps + (uint32_t)K * op.a, a, base, tid);
+          for (int m = 0; m < NR; ++m) lds[lds_at<RB>(t0, rb, m)] = cur.v[m];
+          __syncthreads();
+          lds_offsets<K, RB>(P + K, tid, t0, rb);
 #pragma unroll
-        for (int m = 0; m < (1 << RB); ++m) r.v[m] = lds[ctx.lidx(m)];
-        break;
+          for (int m = 0; m < NR; ++m) cur.v[m] = lds[lds_at<RB>(t0, rb, m)];
+          ctx.setup(nm, base, tid);
+          break;
+        }
+        case TC_SWAPSLOTS:
+          ctx.setup(a.maps + (size_t)K * op.map, base, tid);
+          break;
+        default:
+          break;
       }
-      case TC_SWAPSLOTS: {
-        ctx.setup(a.maps + (uint32_t)K * op.a, a, base, tid);
-        break;
-      }
-      default:
-        break;
+    }
+#pragma unroll
+    for (int m = 0; m < NR; ++m) psi[ctx.gidx(m)] = cur.v[m];
+    if (!PF || !more) break;
+    tile = ntile;
+    if (PF) {
+#pragma unroll
+      for (int m = 0; m < NR; ++m) cur.v[m] = nxt.v[m];
+      base = nbase;
+    } else {
+      base = tile_base(tile, a);
+#pragma unroll
+      for (int m = 0; m < NR; ++m) cur.v[m] = psi[base | c0.gidx(m)];
     }
   }
-#pragma unroll
-  for (int m = 0; m < (1 << RB); ++m) psi[ctx.gidx(m)] = r.v[m];
 }
 
 // =====================================================================================
@@ -787,7 +739,7 @@ inline unsigned grid_for(uint64_t work, int block, unsigned cap = 1u << 20) {
 }
 }  // namespace
 
-int tile_max_bits(int prec) { return prec == 0 ? 12 : 13; }
+int tile_max_bits(int prec) { return 13; }
 int tile_reg_bits(int K) { return K < 4 ? K : 4; }
 
 hipError_t launch_init_basis(int prec, void* psi, uint64_t dim, uint64_t one_at, hipStream_t s) {
@@ -875,40 +827,73 @@ hipError_t launch_diagk(int prec, void* psi, uint64_t dim, const DiagKArgs& a, c
   return hipGetLastError();
 }
 
-hipError_t launch_tile(int prec, int K, void* psi, uint64_t ntiles, const TileArgs& a, hipStream_t s) {
-  if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
-  const unsigned g = (unsigned)ntiles;
-#define QR_TILE_CASE(KK)                                                                       \
-  case KK: {                                                                                   \
-    constexpr int RBv = KK < 4 ? KK : 4;                                                       \
-    constexpr int threads = 1 << (KK - RBv);                                                   \
-    if (prec == 0) {                                                                           \
-      if constexpr (KK <= 12) dev::k_tile<double, KK, RBv><<<g, threads, 0, s>>>((double2*)psi, a); \
-      else return hipErrorInvalidValue;                                                        \
-    } else {                                                                                   \
-      dev::k_tile<float, KK, RBv><<<g, threads, 0, s>>>((float2*)psi, a);                      \
-    }                                                                                          \
-    break;                                                                                     \
+namespace {
+// Workgroups that fit on the device at once for a tile kernel variant (persistent grid).
+template <typename R, int K, int RB, bool PF>
+unsigned tile_grid(uint64_t ntiles) {
+  static unsigned cached = 0;
+  if (!cached) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&dev::k_tile<R, K, RB, PF>),
+                                                       1 << (K - RB), 0);
+    if (cus <= 0) cus = 256;
+    if (per_cu <= 0) per_cu = 1;
+    cached = (unsigned)(cus * per_cu);
   }
-  switch (K) {
-    QR_TILE_CASE(1)
-    QR_TILE_CASE(2)
-    QR_TILE_CASE(3)
-    QR_TILE_CASE(4)
-    QR_TILE_CASE(5)
-    QR_TILE_CASE(6)
-    QR_TILE_CASE(7)
-    QR_TILE_CASE(8)
-    QR_TILE_CASE(9)
-    QR_TILE_CASE(10)
-    QR_TILE_CASE(11)
-    QR_TILE_CASE(12)
-    QR_TILE_CASE(13)
-    default:
-      return hipErrorInvalidValue;
-  }
-#undef QR_TILE_CASE
+  if (!PF) return (unsigned)ntiles;  // one tile per workgroup
+  return (unsigned)std::min<uint64_t>(ntiles, cached);
+}
+}  // namespace
+
+namespace {
+template <typename R, int K, int RB>
+hipError_t launch_tile_t(bool prefetch, void* psi, const TileArgs& a, hipStream_t s) {
+  using C = typename dev::CT<R>::T;
+  constexpr int threads = 1 << (K - RB);
+  if (prefetch)
+    dev::k_tile<R, K, RB, true><<<tile_grid<R, K, RB, true>(a.ntiles), threads, 0, s>>>((C*)psi, a);
+  else
+    dev::k_tile<R, K, RB, false><<<tile_grid<R, K, RB, false>(a.ntiles), threads, 0, s>>>((C*)psi, a);
   return hipGetLastError();
+}
+
+template <typename R>
+hipError_t launch_tile_r(int K, int RB, bool prefetch, void* psi, const TileArgs& a, hipStream_t s) {
+  if (RB == 5) {
+    switch (K) {
+      case 10: return launch_tile_t<R, 10, 5>(prefetch, psi, a, s);
+      case 11: return launch_tile_t<R, 11, 5>(prefetch, psi, a, s);
+      case 12: return launch_tile_t<R, 12, 5>(prefetch, psi, a, s);
+      case 13: return launch_tile_t<R, 13, 5>(prefetch, psi, a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
+  if (RB != tile_reg_bits(K)) return hipErrorInvalidValue;
+  switch (K) {
+    case 1: return launch_tile_t<R, 1, 1>(prefetch, psi, a, s);
+    case 2: return launch_tile_t<R, 2, 2>(prefetch, psi, a, s);
+    case 3: return launch_tile_t<R, 3, 3>(prefetch, psi, a, s);
+    case 4: return launch_tile_t<R, 4, 4>(prefetch, psi, a, s);
+    case 5: return launch_tile_t<R, 5, 4>(prefetch, psi, a, s);
+    case 6: return launch_tile_t<R, 6, 4>(prefetch, psi, a, s);
+    case 7: return launch_tile_t<R, 7, 4>(prefetch, psi, a, s);
+    case 8: return launch_tile_t<R, 8, 4>(prefetch, psi, a, s);
+    case 9: return launch_tile_t<R, 9, 4>(prefetch, psi, a, s);
+    case 10: return launch_tile_t<R, 10, 4>(prefetch, psi, a, s);
+    case 11: return launch_tile_t<R, 11, 4>(prefetch, psi, a, s);
+    case 12: return launch_tile_t<R, 12, 4>(prefetch, psi, a, s);
+    case 13: return launch_tile_t<R, 13, 4>(prefetch, psi, a, s);
+    default: return hipErrorInvalidValue;
+  }
+}
+}  // namespace
+
+hipError_t launch_tile(int prec, int K, int RB, bool prefetch, void* psi, const TileArgs& a, hipStream_t s) {
+  if (a.ntiles == 0) return hipErrorInvalidValue;
+  return prec == 0 ? launch_tile_r<double>(K, RB, prefetch, psi, a, s)
+                   : launch_tile_r<float>(K, RB, prefetch, psi, a, s);
 }
 
 hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const double* u, uint64_t shots,

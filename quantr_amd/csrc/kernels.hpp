@@ -43,21 +43,26 @@ enum TileCode : uint32_t {
   TC_SWAPSLOTS = 4,  // a = index of the new slot map (relabel, no data movement)
 };
 
+// One interpreted op of a tile program (32 bytes, read with scalar loads).
 struct TileOpDev {
-  uint32_t code, a;
-  uint64_t cmask;
-  uint32_t coef, b, c, d;
+  uint32_t w0;          // code | a << 8 | b << 16 | flags << 24
+  uint32_t ns;          // register slots that must be 1
+  uint64_t lane_cmask;  // control bits on thread slots / outside the tile
+  uint32_t map;         // REMAP / SWAPSLOTS: new map index
+  uint32_t swz;         // REMAP: swizzle table index
+  uint32_t pad[2];
 };
 static_assert(sizeof(TileOpDev) == 32, "TileOpDev layout");
 
 struct TileArgs {
   const TileOpDev* ops;
-  const uint8_t* maps;
-  const double2* coefs;
+  const double2* coefs;   // 4 per op, op o at coefs[4 o]
+  const uint32_t* maps;   // per map, K words: global bit | tile position << 8
+  const uint32_t* swz;    // per remap, 2K words: LDS pattern of each slot, old map then new map
+  uint64_t ntiles;
+  uint64_t nontile_mask;  // index bits outside the tile; tile number bits are deposited there
   uint32_t nops;
-  int nnontile;
-  uint8_t tbits[16];
-  uint8_t nontile[48];
+  uint32_t pad;
 };
 
 struct BlockJob {
@@ -67,7 +72,7 @@ struct BlockJob {
 };
 
 // Tile kernel variants available: tile bits K and register bits RB per precision.
-int tile_max_bits(int prec);  // 12 for complex128, 13 for complex64
+int tile_max_bits(int prec);  // widest tile variant compiled (13)
 int tile_reg_bits(int K);
 
 // ---- launchers (prec: 0 = complex128, 1 = complex64) ----
@@ -85,7 +90,7 @@ hipError_t launch_densek(int prec, void* psi, void* tmp, uint64_t tmp_groups, ui
                          const DenseKArgs& a, const double2* M, const uint64_t* off, hipStream_t s);
 hipError_t launch_diagk(int prec, void* psi, uint64_t dim, const DiagKArgs& a, const double2* D,
                         hipStream_t s);
-hipError_t launch_tile(int prec, int K, void* psi, uint64_t ntiles, const TileArgs& a, hipStream_t s);
+hipError_t launch_tile(int prec, int K, int RB, bool prefetch, void* psi, const TileArgs& a, hipStream_t s);
 hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const double* u, uint64_t shots,
                               uint64_t* out, hipStream_t s);
 hipError_t launch_block_mass(int prec, const void* psi, uint64_t dim, int bb, double* mass, hipStream_t s);

@@ -98,6 +98,63 @@ uint64_t greedy_bits(uint64_t base, int want, const std::vector<int>& cands,
   return cur;
 }
 
+// True if the low `bits` bits of the patterns of the first `m` thread slots are linearly
+// independent over GF(2).
+bool independent(const std::vector<uint16_t>& pat, int m, int bits) {
+  const uint16_t mask = (uint16_t)((1u << bits) - 1);
+  uint16_t basis[16] = {0};
+  int rank = 0;
+  for (int i = 0; i < m; ++i) {
+    uint16_t v = pat[i] & mask;
+    for (int b = bits - 1; b >= 0 && v; --b) {
+      if (!((v >> b) & 1)) continue;
+      if (!basis[b]) {
+        basis[b] = v;
+        ++rank;
+        v = 0;
+        break;
+      }
+      v ^= basis[b];
+    }
+  }
+  return rank == m;
+}
+
+// LDS address pattern per tile position: S[p] = (1 << p) ^ e_p with e_p in the low 4 bits
+// for p >= 4 (an invertible linear map). Chosen so that lane slots 0..3 of both maps land on
+// distinct 16-byte slots (low 3 and low 4 address bits independent).
+std::vector<uint16_t> choose_swizzle(int k, int nts, const std::vector<int>& oldm, const std::vector<int>& newm) {
+  std::vector<uint16_t> S(k);
+  for (int p = 0; p < k; ++p) S[p] = (uint16_t)(1u << p);
+  const int lanes = std::min(nts, 4);
+  if (lanes == 0 || k <= 4) return S;
+  auto ok = [&]() {
+    for (const std::vector<int>* m : {&oldm, &newm}) {
+      std::vector<uint16_t> pat(lanes);
+      for (int i = 0; i < lanes; ++i) pat[i] = S[(*m)[i]];
+      if (!independent(pat, std::min(lanes, 3), 3)) return false;
+      if (!independent(pat, lanes, 4)) return false;
+    }
+    return true;
+  };
+  if (ok()) return S;
+  std::vector<int> free_pos;
+  for (const std::vector<int>* m : {&oldm, &newm})
+    for (int i = 0; i < lanes; ++i)
+      if ((*m)[i] >= 4 && std::find(free_pos.begin(), free_pos.end(), (*m)[i]) == free_pos.end())
+        free_pos.push_back((*m)[i]);
+  uint64_t x = 0x9E3779B97F4A7C15ull;
+  for (int attempt = 0; attempt < 4096; ++attempt) {
+    for (int p : free_pos) {
+      x ^= x << 13; x ^= x >> 7; x ^= x << 17;
+      S[p] = (uint16_t)((1u << p) ^ (x & 15));
+    }
+    if (ok()) return S;
+  }
+  for (int p = 0; p < k; ++p) S[p] = (uint16_t)(1u << p);
+  return S;
+}
+
 }  // namespace
 
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
@@ -193,6 +250,31 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         if (map[sl] == p) return sl;
       return -1;
     };
+    // Control bits held in register slots become a uniform slot mask; the rest (thread slots
+    // and bits outside the tile) a per-lane mask on the global index.
+    auto split_controls = [&](uint64_t cmask, TileOpHost& t) {
+      for (int b = 0; b < 64; ++b) {
+        if (!(cmask & (1ull << b))) continue;
+        const int p = pos_of_bit[b];
+        const int sl = p >= 0 ? slot_of_pos(p) : -1;
+        if (sl >= nts) t.ns |= 1u << (sl - nts);
+        else t.lane_cmask |= 1ull << b;
+      }
+    };
+    // A remap moves data through LDS; its swizzle keeps the 16 lanes of a quarter-wave on
+    // distinct 16-byte LDS slots for both the write (old map) and the read (new map).
+    auto emit_remap = [&](const std::vector<int>& nm) {
+      TileOpHost t{};
+      t.code = TC_REMAP;
+      t.swz = (uint32_t)(s.swz.size() / (2 * k));
+      std::vector<uint16_t> table = choose_swizzle(k, nts, map, nm);
+      // per-slot LDS patterns: old map (write) then new map (read)
+      for (int sl = 0; sl < k; ++sl) s.swz.push_back(table[map[sl]]);
+      for (int sl = 0; sl < k; ++sl) s.swz.push_back(table[nm[sl]]);
+      t.a = (uint32_t)push_map(nm);
+      s.ops.push_back(t);
+      map = nm;
+    };
 
     while (!rem.empty()) {
       // global bits currently in register slots
@@ -228,13 +310,12 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       for (int i : now) {
         const FastOp& op = ops[i];
         TileOpHost t{};
-        t.cmask = op.cmask;
+        split_controls(op.cmask, t);
         if (op.kind == FK_DENSE1) {
           t.code = TC_U1;
           const int p = pos_of_bit[op.bits[0]];
           t.a = (uint32_t)(slot_of_pos(p) - nts);
-          t.coef = (uint32_t)s.coefs.size();
-          s.coefs.insert(s.coefs.end(), op.m.begin(), op.m.begin() + 4);
+          for (int q = 0; q < 4; ++q) t.coef[q] = op.m[q];
           int form = op.form;
           if (form == U1_PERM && op.m[1] == cd(1, 0) && op.m[2] == cd(1, 0)) form = 3;  // X
           t.b = (uint32_t)form;
@@ -245,10 +326,14 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           const int p = pos_of_bit[op.bits[0]];
           const int sl = p >= 0 ? slot_of_pos(p) : -1;
           t.b = (sl >= nts) ? (uint32_t)(sl - nts) : 0xFFu;
-          t.coef = (uint32_t)s.coefs.size();
-          s.coefs.push_back(op.m[0]);
-          s.coefs.push_back(op.m[1]);
-          t.c = (op.m[0] == cd(1, 0)) ? 1u : 0u;  // only the |1> half changes
+          t.coef[0] = op.m[0];
+          t.coef[1] = op.m[1];
+          if (op.m[0] == cd(1, 0)) {
+            t.flags = 1;  // only the |1> half changes
+            if (op.m[1] == cd(-1, 0)) t.flags |= 1u << 4;
+            else if (op.m[1] == cd(0, 1)) t.flags |= 2u << 4;
+            else if (op.m[1] == cd(0, -1)) t.flags |= 3u << 4;
+          }
           s.ops.push_back(t);
         } else {  // swap: exchange the tile positions held by the two slots
           const int p0 = pos_of_bit[op.bits[0]], p1 = pos_of_bit[op.bits[1]];
@@ -267,12 +352,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         if (__builtin_popcountll(R2) < r) R2 |= 1ull << b;
       if (now.empty() && count_absorb(rem, mk, eligible, R2) == 0)
         throw Error{QR_ERR_UNSUPPORTED, "internal planner error: no progress inside a tile"};
-      std::vector<int> nm = make_map(R2);
-      TileOpHost t{};
-      t.code = TC_REMAP;
-      t.a = (uint32_t)push_map(nm);
-      s.ops.push_back(t);
-      map = nm;
+      emit_remap(make_map(R2));
     }
     // ---- final layout: low tile positions on the low thread slots for a coalesced store --
     bool canon = true;
@@ -282,12 +362,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       uint64_t Rf = 0;
       for (int b : nonlow_bits)
         if (__builtin_popcountll(Rf) < r) Rf |= 1ull << b;
-      std::vector<int> nm = make_map(Rf);
-      TileOpHost t{};
-      t.code = TC_REMAP;
-      t.a = (uint32_t)push_map(nm);
-      s.ops.push_back(t);
-      map = nm;
+      emit_remap(make_map(Rf));
     }
     stages.push_back(std::move(s));
     remaining = left;

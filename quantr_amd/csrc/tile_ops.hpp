@@ -1,0 +1,167 @@
+// tile_ops.hpp — register-level gate kernels shared by the interpreted tile sweep
+// (kernels.hip) and the runtime-specialised tile sweeps (jit.cpp compiles this file's text
+// with hipRTC). Self-contained: no includes, only HIP built-ins.
+#pragma once
+
+namespace qr {
+namespace dev {
+
+template <typename R>
+struct CT;
+template <>
+struct CT<double> {
+  using T = double2;
+};
+template <>
+struct CT<float> {
+  using T = float2;
+};
+
+
+template <typename R, int RB>
+struct Regs {
+  typename CT<R>::T v[1 << RB];
+};
+
+// Coefficients travel by value (never through a pointer to a local array, which would put
+// them in scratch memory).
+struct Coef4 {
+  double2 c[4];
+};
+
+// ---- complex helpers -------------------------------------------------------------------
+// factor forms of a diagonal entry: 0 general, 1 = -1, 2 = +i, 3 = -i
+template <typename R, int FORM>
+__device__ __forceinline__ void cscale(typename CT<R>::T& x, R fr, R fi) {
+  const R xr = x.x, xi = x.y;
+  if (FORM == 1) {
+    x.x = -xr;
+    x.y = -xi;
+  } else if (FORM == 2) {
+    x.x = -xi;
+    x.y = xr;
+  } else if (FORM == 3) {
+    x.x = xi;
+    x.y = -xr;
+  } else {
+    x.x = fma(fr, xr, -fi * xi);
+    x.y = fma(fr, xi, fi * xr);
+  }
+}
+
+// 2x2 on register slot J for every register pair whose register-held controls (ns) are 1.
+template <typename R, int RB, int J, int FORM, bool HAS_NS>
+__device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const Coef4 u, uint32_t ns) {
+  const R u00r = (R)u.c[0].x, u00i = (R)u.c[0].y, u01r = (R)u.c[1].x, u01i = (R)u.c[1].y;
+  const R u10r = (R)u.c[2].x, u10i = (R)u.c[2].y, u11r = (R)u.c[3].x, u11i = (R)u.c[3].y;
+#pragma unroll
+  for (int m0 = 0; m0 < (1 << RB); ++m0) {
+    if (m0 & (1 << J)) continue;
+    if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
+    const int m1 = m0 | (1 << J);
+    const R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
+    R o0r, o0i, o1r, o1i;
+    if (FORM == 3) {  // X: swap
+      o0r = yr; o0i = yi; o1r = xr; o1i = xi;
+    } else if (FORM == 1) {  // anti-diagonal
+      o0r = fma(u01r, yr, -u01i * yi);
+      o0i = fma(u01r, yi, u01i * yr);
+      o1r = fma(u10r, xr, -u10i * xi);
+      o1i = fma(u10r, xi, u10i * xr);
+    } else if (FORM == 2) {  // real
+      o0r = fma(u00r, xr, u01r * yr);
+      o0i = fma(u00r, xi, u01r * yi);
+      o1r = fma(u10r, xr, u11r * yr);
+      o1i = fma(u10r, xi, u11r * yi);
+    } else {
+      o0r = u00r * xr; o0i = u00r * xi;
+      o0r = fma(-u00i, xi, o0r); o0i = fma(u00i, xr, o0i);
+      o0r = fma(u01r, yr, o0r); o0i = fma(u01r, yi, o0i);
+      o0r = fma(-u01i, yi, o0r); o0i = fma(u01i, yr, o0i);
+      o1r = u10r * xr; o1i = u10r * xi;
+      o1r = fma(-u10i, xi, o1r); o1i = fma(u10i, xr, o1i);
+      o1r = fma(u11r, yr, o1r); o1i = fma(u11r, yi, o1i);
+      o1r = fma(-u11i, yi, o1r); o1i = fma(u11i, yr, o1i);
+    }
+    a.v[m0].x = o0r; a.v[m0].y = o0i;
+    a.v[m1].x = o1r; a.v[m1].y = o1i;
+  }
+}
+
+template <typename R, int RB, int J, bool HAS_NS>
+__device__ __forceinline__ void u1_form(Regs<R, RB>& a, const Coef4 u, uint32_t form, uint32_t ns) {
+  switch (form) {
+    case 3: u1_slot<R, RB, J, 3, HAS_NS>(a, u, ns); break;
+    case 1: u1_slot<R, RB, J, 1, HAS_NS>(a, u, ns); break;
+    case 2: u1_slot<R, RB, J, 2, HAS_NS>(a, u, ns); break;
+    default: u1_slot<R, RB, J, 0, HAS_NS>(a, u, ns); break;
+  }
+}
+
+template <typename R, int RB, bool HAS_NS>
+__device__ __forceinline__ void u1_dispatch(Regs<R, RB>& a, uint32_t slot, const Coef4 u, uint32_t form,
+                                            uint32_t ns) {
+  switch (slot) {
+    case 0: u1_form<R, RB, 0, HAS_NS>(a, u, form, ns); break;
+    case 1: if constexpr (RB > 1) u1_form<R, RB, 1, HAS_NS>(a, u, form, ns); break;
+    case 2: if constexpr (RB > 2) u1_form<R, RB, 2, HAS_NS>(a, u, form, ns); break;
+    case 3: if constexpr (RB > 3) u1_form<R, RB, 3, HAS_NS>(a, u, form, ns); break;
+    case 4: if constexpr (RB > 4) u1_form<R, RB, 4, HAS_NS>(a, u, form, ns); break;
+    default: break;
+  }
+}
+
+// diag on register slot J: registers with bit J = 1 get d1, the others d0 (skipped if ONLY1).
+template <typename R, int RB, int J, bool ONLY1, int FORM, bool HAS_NS>
+__device__ __forceinline__ void d1_reg(Regs<R, RB>& a, const Coef4 d, uint32_t ns) {
+  const R d0r = (R)d.c[0].x, d0i = (R)d.c[0].y, d1r = (R)d.c[1].x, d1i = (R)d.c[1].y;
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) {
+    if (HAS_NS && ((uint32_t)m & ns) != ns) continue;
+    const bool bit = (m >> J) & 1;
+    if (ONLY1 && !bit) continue;
+    if (bit) cscale<R, ONLY1 ? FORM : 0>(a.v[m], d1r, d1i);
+    else cscale<R, 0>(a.v[m], d0r, d0i);
+  }
+}
+
+template <typename R, int RB, int J, bool HAS_NS>
+__device__ __forceinline__ void d1_reg_form(Regs<R, RB>& a, const Coef4 d, uint32_t flags, uint32_t ns) {
+  if (!(flags & 1)) {
+    d1_reg<R, RB, J, false, 0, HAS_NS>(a, d, ns);
+    return;
+  }
+  switch (flags >> 4) {
+    case 1: d1_reg<R, RB, J, true, 1, HAS_NS>(a, d, ns); break;
+    case 2: d1_reg<R, RB, J, true, 2, HAS_NS>(a, d, ns); break;
+    case 3: d1_reg<R, RB, J, true, 3, HAS_NS>(a, d, ns); break;
+    default: d1_reg<R, RB, J, true, 0, HAS_NS>(a, d, ns); break;
+  }
+}
+
+template <typename R, int RB, bool HAS_NS>
+__device__ __forceinline__ void d1_reg_dispatch(Regs<R, RB>& a, uint32_t slot, const Coef4 d, uint32_t flags,
+                                                uint32_t ns) {
+  switch (slot) {
+    case 0: d1_reg_form<R, RB, 0, HAS_NS>(a, d, flags, ns); break;
+    case 1: if constexpr (RB > 1) d1_reg_form<R, RB, 1, HAS_NS>(a, d, flags, ns); break;
+    case 2: if constexpr (RB > 2) d1_reg_form<R, RB, 2, HAS_NS>(a, d, flags, ns); break;
+    case 3: if constexpr (RB > 3) d1_reg_form<R, RB, 3, HAS_NS>(a, d, flags, ns); break;
+    case 4: if constexpr (RB > 4) d1_reg_form<R, RB, 4, HAS_NS>(a, d, flags, ns); break;
+    default: break;
+  }
+}
+
+// every register (whose register-held controls hold) times one factor
+template <typename R, int RB, int FORM, bool HAS_NS>
+__device__ __forceinline__ void scale_all(Regs<R, RB>& a, R fr, R fi, uint32_t ns) {
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) {
+    if (HAS_NS && ((uint32_t)m & ns) != ns) continue;
+    cscale<R, FORM>(a.v[m], fr, fi);
+  }
+}
+
+
+}  // namespace dev
+}  // namespace qr

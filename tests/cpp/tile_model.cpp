@@ -51,7 +51,8 @@ static void apply_standalone(uint32_t n, std::vector<cdv>& psi, const FastOp& op
   }
 }
 
-// One tile stage, modelled thread by thread exactly as k_tile computes indices.
+// One tile stage, modelled thread by thread exactly as k_tile computes indices (maps with
+// global bits and tile positions, per-lane / register-slot control split, swizzled LDS).
 static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB) {
   const int K = s.k, NT = K - RB;
   const uint64_t ntiles = 1ull << (n - K);
@@ -62,104 +63,75 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB) 
     if (!(tm & (1ull << b))) nontile.push_back((int)b);
   const int T = 1 << NT, R = 1 << RB;
   std::vector<cdv> lds(1u << K);
-  struct Ctx {
-    uint64_t g0;
-    uint32_t l0;
-    uint64_t gb[8];
-    uint32_t lb[8];
-    uint64_t regmask;
-  };
-  auto setup = [&](const int* mp, uint64_t base, uint32_t tid, Ctx& c) {
-    c.g0 = base;
-    c.l0 = 0;
-    for (int sl = 0; sl < NT; ++sl)
-      if ((tid >> sl) & 1) {
-        c.g0 |= 1ull << s.tbits[mp[sl]];
-        c.l0 |= 1u << mp[sl];
-      }
-    c.regmask = 0;
-    for (int j = 0; j < RB; ++j) {
-      c.gb[j] = 1ull << s.tbits[mp[NT + j]];
-      c.lb[j] = 1u << mp[NT + j];
-      c.regmask |= c.gb[j];
-    }
-  };
-  auto gidx = [&](const Ctx& c, int m) {
-    uint64_t x = c.g0;
-    for (int j = 0; j < RB; ++j)
-      if ((m >> j) & 1) x |= c.gb[j];
-    return x;
-  };
-  auto lidx = [&](const Ctx& c, int m) {
-    uint32_t x = c.l0;
-    for (int j = 0; j < RB; ++j)
-      if ((m >> j) & 1) x |= c.lb[j];
-    return x;
-  };
+  std::vector<uint8_t> lds_written(1u << K);
+  auto gbit = [&](int mapi, int slot) { return s.tbits[s.maps[(size_t)mapi * K + slot]]; };
   for (uint64_t blk = 0; blk < ntiles; ++blk) {
     uint64_t base = 0;
     for (size_t i = 0; i < nontile.size(); ++i) base |= ((blk >> i) & 1ull) << nontile[i];
-    std::vector<Ctx> ctx(T);
+    int cur_map = 0;
+    auto gidx = [&](int t, int m) {
+      uint64_t x = base;
+      for (int sl = 0; sl < NT; ++sl)
+        if ((t >> sl) & 1) x |= 1ull << gbit(cur_map, sl);
+      for (int j = 0; j < RB; ++j)
+        if ((m >> j) & 1) x |= 1ull << gbit(cur_map, NT + j);
+      return x;
+    };
     std::vector<std::vector<cdv>> reg(T, std::vector<cdv>(R));
-    for (int t = 0; t < T; ++t) {
-      setup(&s.maps[0], base, t, ctx[t]);
-      for (int m = 0; m < R; ++m) reg[t][m] = psi[gidx(ctx[t], m)];
-    }
+    for (int t = 0; t < T; ++t)
+      for (int m = 0; m < R; ++m) reg[t][m] = psi[gidx(t, m)];
     for (const TileOpHost& op : s.ops) {
-      switch (op.code) {
-        case TC_U1: {
-          const cdv* u = &s.coefs[op.coef];
-          for (int t = 0; t < T; ++t) {
-            const uint64_t need = op.cmask & ~ctx[t].g0;
-            const bool ok = (need & ~ctx[t].regmask) == 0;
-            uint32_t ns = 0;
-            for (int j = 0; j < RB; ++j)
-              if (need & ctx[t].gb[j]) ns |= 1u << j;
-            const int J = (int)op.a;
-            for (int m0 = 0; m0 < R; ++m0) {
-              if (m0 & (1 << J)) continue;
-              if (!ok || ((uint32_t)m0 & ns) != ns) continue;
-              const int m1 = m0 | (1 << J);
-              const cdv x = reg[t][m0], y = reg[t][m1];
-              reg[t][m0] = u[0] * x + u[1] * y;
-              reg[t][m1] = u[2] * x + u[3] * y;
-            }
+      for (int t = 0; t < T && (op.code == TC_U1 || op.code == TC_D1); ++t) {
+        const uint64_t g0 = gidx(t, 0);
+        const bool lane_ok = (g0 & op.lane_cmask) == op.lane_cmask;
+        if (!lane_ok) continue;
+        if (op.code == TC_U1) {
+          const int J = (int)op.a;
+          for (int m0 = 0; m0 < R; ++m0) {
+            if (m0 & (1 << J)) continue;
+            if (((uint32_t)m0 & op.ns) != op.ns) continue;
+            const int m1 = m0 | (1 << J);
+            const cdv x = reg[t][m0], y = reg[t][m1];
+            reg[t][m0] = op.coef[0] * x + op.coef[1] * y;
+            reg[t][m1] = op.coef[2] * x + op.coef[3] * y;
           }
-          break;
-        }
-        case TC_D1: {
-          const cdv* d = &s.coefs[op.coef];
-          for (int t = 0; t < T; ++t) {
-            const uint64_t need = op.cmask & ~ctx[t].g0;
-            const bool ok = (need & ~ctx[t].regmask) == 0;
-            uint32_t ns = 0;
-            for (int j = 0; j < RB; ++j)
-              if (need & ctx[t].gb[j]) ns |= 1u << j;
-            const bool lane_bit = (ctx[t].g0 >> op.a) & 1;
-            for (int m = 0; m < R; ++m) {
-              if (((uint32_t)m & ns) != ns || !ok) continue;
-              const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : lane_bit;
-              reg[t][m] *= bit ? d[1] : d[0];
-            }
+        } else {
+          for (int m = 0; m < R; ++m) {
+            if (((uint32_t)m & op.ns) != op.ns) continue;
+            const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : ((g0 >> op.a) & 1);
+            if ((op.flags & 1) && !bit) continue;
+            reg[t][m] *= bit ? op.coef[1] : op.coef[0];
           }
-          break;
         }
-        case TC_REMAP: {
-          for (int t = 0; t < T; ++t)
-            for (int m = 0; m < R; ++m) lds[lidx(ctx[t], m)] = reg[t][m];
-          for (int t = 0; t < T; ++t) {
-            setup(&s.maps[(size_t)K * op.a], base, t, ctx[t]);
-            for (int m = 0; m < R; ++m) reg[t][m] = lds[lidx(ctx[t], m)];
+      }
+      if (op.code == TC_REMAP) {
+        const uint16_t* P = &s.swz[(size_t)op.swz * 2 * K];
+        auto lofs = [&](int which, int t, int m) {
+          const uint16_t* Q = P + which * K;
+          uint32_t x = 0;
+          for (int sl = 0; sl < NT; ++sl)
+            if ((t >> sl) & 1) x ^= Q[sl];
+          for (int j = 0; j < RB; ++j)
+            if ((m >> j) & 1) x ^= Q[NT + j];
+          return x;
+        };
+        std::fill(lds_written.begin(), lds_written.end(), 0);
+        for (int t = 0; t < T; ++t)
+          for (int m = 0; m < R; ++m) {
+            const uint32_t at = lofs(0, t, m);
+            if (at >= lds.size() || lds_written[at]) throw Error{QR_ERR_HIP, "model: LDS write collision"};
+            lds_written[at] = 1;
+            lds[at] = reg[t][m];
           }
-          break;
-        }
-        case TC_SWAPSLOTS:
-          for (int t = 0; t < T; ++t) setup(&s.maps[(size_t)K * op.a], base, t, ctx[t]);
-          break;
+        for (int t = 0; t < T; ++t)
+          for (int m = 0; m < R; ++m) reg[t][m] = lds[lofs(1, t, m)];
+        cur_map = (int)op.a;
+      } else if (op.code == TC_SWAPSLOTS) {
+        cur_map = (int)op.a;
       }
     }
     for (int t = 0; t < T; ++t)
-      for (int m = 0; m < R; ++m) psi[gidx(ctx[t], m)] = reg[t][m];
+      for (int m = 0; m < R; ++m) psi[gidx(t, m)] = reg[t][m];
   }
 }
 
@@ -167,7 +139,7 @@ extern "C" {
 
 // Plans `prog` with the engine's planner (tile width k, register bits, low bits) and executes
 // the plan on the CPU model. state: 2^n complex128, in/out. Returns 0, or 1 with a message.
-int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, int low_bits,
+int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, int low_bits, int reg_bits,
               uint64_t* sweeps_out) {
   try {
     auto gates = lower_exact(n, prog);
@@ -175,6 +147,7 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
     PlanParams pp;
     pp.tile_bits = std::min<int>(tile_bits, (int)n);
     pp.reg_bits = pp.tile_bits < 4 ? pp.tile_bits : 4;
+    if (reg_bits == 5 && pp.tile_bits >= 10) pp.reg_bits = 5;
     pp.low_bits = low_bits;
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);

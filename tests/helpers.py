@@ -60,6 +60,15 @@ def oracle_simulate(n, program, register=None, method="sv", threads=0):
     return state
 
 
+def oracle_apply_inplace(n, state, program, threads=0):
+    """Applies `program` to `state` (complex128, 2^n) in place with the oracle state vector."""
+    ops, ctl, img, some = program.arrays()
+    rc = oracle().orc_simulate_sv(n, state.ctypes.data, _arr(ops), len(ops), _arr(ctl), len(ctl), _arr(img),
+                                  len(img), _arr(some), len(some), threads)
+    if rc != 0:
+        raise RuntimeError(oracle().orc_last_error().decode())
+
+
 def oracle_measure(n, state, uniforms, literal=False):
     u = np.ascontiguousarray(uniforms, dtype=np.float64)
     out = np.empty(u.size, dtype=np.uint64)

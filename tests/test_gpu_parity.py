@@ -30,9 +30,9 @@ def need_gpu():
         pytest.fail("no GPU visible: the HIP engine cannot run")
 
 
-def run_gpu(n, program, fused=1, precision=_capi.QR_C128, register=None, tile_bits=0):
+def run_gpu(n, program, fused=1, precision=_capi.QR_C128, register=None, tile_bits=0, jit=0):
     st = _capi.State(n, precision)
-    st.set_options(fused=fused, tile_bits=tile_bits)
+    st.set_options(fused=fused, tile_bits=tile_bits, jit=jit)
     if register is not None:
         st.set_amplitudes(register)
     st.apply(program)
@@ -258,4 +258,32 @@ def test_register_dimension_error():
         st.set_amplitudes(np.zeros(4, dtype=np.complex128))
     assert str(e.value) == ("The custom register has a product state dimension of 2, while the number of "
                             "qubits is 3. These must equal each other.")
+    st.close()
+
+
+def test_compiled_stages_random():
+    """Tile stages compiled at run time (hipRTC) against the oracle."""
+    rng = np.random.default_rng(606)
+    for trial in range(8):
+        n = int(rng.integers(6, 17))
+        gates, customs = H.random_gates(n, 80, rng, with_custom=(trial % 2 == 0))
+        say("jit trial", trial, "n", n)
+        p = H.to_program(gates, customs)
+        want = H.oracle_simulate(n, p)
+        for prec, tol in ((_capi.QR_C128, TOL_FUSED), (_capi.QR_C64, TOL_C64)):
+            got = run_gpu(n, p, fused=1, precision=prec, jit=1)
+            assert np.max(np.abs(got - want)) < tol, (trial, prec)
+
+
+def test_compiled_stages_brickwork_repeat():
+    """Second application of the same structure switches to compiled stages (jit = 2)."""
+    n = 18
+    p = W.to_program(W.brickwork(n, depth=20, seed=77))
+    want = H.oracle_simulate(n, p, threads=8)
+    st = _capi.State(n)
+    for rep in range(3):
+        st.reset()
+        st.apply(p)
+        got = st.amplitudes()
+        assert np.max(np.abs(got - want)) < TOL_FUSED, rep
     st.close()

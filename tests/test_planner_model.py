@@ -30,18 +30,19 @@ def model():
                                "--offload-arch=gfx950"] + srcs + ["-o", SO],
                               stdout=subprocess.DEVNULL, stderr=subprocess.DEVNULL)
     lib = C.CDLL(SO)
-    lib.model_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_uint64)]
+    lib.model_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                              C.POINTER(C.c_uint64)]
     lib.model_error.restype = C.c_char_p
     return lib
 
 
-def run_model(n, program, tile_bits, low_bits=4):
+def run_model(n, program, tile_bits, low_bits=4, reg_bits=4):
     lib = model()
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[0] = 1.0
     p = program.c_struct()
     sw = C.c_uint64()
-    rc = lib.model_run(n, psi.ctypes.data, C.byref(p), tile_bits, low_bits, C.byref(sw))
+    rc = lib.model_run(n, psi.ctypes.data, C.byref(p), tile_bits, low_bits, reg_bits, C.byref(sw))
     assert rc == 0, lib.model_error()
     return psi, sw.value
 
@@ -79,3 +80,15 @@ def test_model_swap_heavy():
         p = H.to_program(gates)
         got, _ = run_model(n, p, min(n, 7))
         assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12, trial
+
+
+@pytest.mark.parametrize("tile_bits", [10, 11, 12, 13])
+def test_model_five_register_bits(tile_bits):
+    rng = np.random.default_rng(2000 + tile_bits)
+    for trial in range(4):
+        n = int(rng.integers(tile_bits, 15))
+        gates, customs = H.random_gates(n, 60, rng, with_custom=(trial % 2 == 0))
+        p = H.to_program(gates, customs)
+        got, _ = run_model(n, p, tile_bits, low_bits=3, reg_bits=5)
+        want = H.oracle_simulate(n, p)
+        assert np.max(np.abs(got - want)) < 1e-12, (trial, n)
