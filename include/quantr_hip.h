@@ -144,7 +144,7 @@ typedef struct qr_options {
                             the fold); 2 (default): exact when n <= 26, else parallel   */
   uint32_t tile_variant; /* 0: auto; 1: one tile in registers per workgroup;
                             2: next tile prefetched into registers (double buffer)       */
-  uint32_t reg_bits;     /* amplitudes per thread = 2^reg_bits (0 = auto, 4 or 5)       */
+  uint32_t reg_bits;     /* amplitudes per thread = 2^reg_bits (0 = auto, 3, 4 or 5)    */
   uint32_t jit;          /* tile sweeps compiled for the circuit at run time (hipRTC):
                             0 never; 1 always; 2 (default) from the second application of
                             the same circuit structure on this state, or at once for n >= 28 */

@@ -163,7 +163,7 @@ PlanParams plan_params(const qr_state* st) {
   PlanParams pp;
   pp.tile_bits = pick_tile_bits(st);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
-  if (st->opt.reg_bits == 5 && pp.tile_bits >= 10) pp.reg_bits = 5;
+  if ((st->opt.reg_bits == 5 || st->opt.reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = (int)st->opt.reg_bits;
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
   return pp;
 }
@@ -251,8 +251,9 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
   // ---- runtime-specialised stages --------------------------------------------------------
   std::vector<std::string> jit_src;
   bool use_jit = false;
-  if (fused && st->opt.jit != 0 && st->opt.tile_variant != 2) {
-    jit_src = jit_stage_sources(stages, st->prec, pp.reg_bits);
+  const bool persistent = st->opt.tile_variant == 2;
+  if (fused && st->opt.jit != 0) {
+    jit_src = jit_stage_sources(stages, st->prec, pp.reg_bits, persistent);
     size_t h = 0;
     for (const auto& x : jit_src) h = h * 1000003u ^ std::hash<std::string>()(x);
     if (st->opt.jit == 1 || st->n >= 28) {
@@ -286,7 +287,8 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
       a.nontile_mask = (n >= 64 ? ~0ull : ((1ull << n) - 1)) & ~tm;
       ProfScope ps_(st, "tile_sweep", s.bytes);
       if (use_jit)
-        hip_check(jit_launch(jit_src[si], s.k, pp.reg_bits, st->amp, a.coefs, a.ntiles, a.nontile_mask, st->stream),
+        hip_check(jit_launch(jit_src[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles, a.nontile_mask,
+                             st->stream),
                   "compiled tile kernel");
       else
         hip_check(launch_tile(st->prec, s.k, pp.reg_bits, st->opt.tile_variant == 2, st->amp, a, st->stream),
@@ -706,7 +708,7 @@ qr_status qr_state_set_options(qr_state* st, const qr_options* o) {
     if (o->tile_bits > 16) throw Error{QR_ERR_ARG, "tile_bits must be <= 16"};
     if (o->low_bits > 8) throw Error{QR_ERR_ARG, "low_bits must be <= 8"};
     if (o->tile_variant > 2) throw Error{QR_ERR_ARG, "tile_variant must be 0, 1 or 2"};
-    if (o->reg_bits != 0 && o->reg_bits != 4 && o->reg_bits != 5) throw Error{QR_ERR_ARG, "reg_bits must be 0, 4 or 5"};
+    if (o->reg_bits != 0 && (o->reg_bits < 3 || o->reg_bits > 5)) throw Error{QR_ERR_ARG, "reg_bits must be 0, 3, 4 or 5"};
     if (o->jit > 2) throw Error{QR_ERR_ARG, "jit must be 0, 1 or 2"};
     st->opt = *o;
   });

@@ -45,9 +45,13 @@ std::string hex64(uint64_t v) {
   return b;
 }
 
-// Source of one stage kernel. `prec` 0 = complex128, 1 = complex64.
-std::string stage_source(const Stage& s, int prec, int RB) {
+// Source of one stage kernel. `prec` 0 = complex128, 1 = complex64. With `persistent`, each
+// workgroup walks tiles blockIdx.x, +gridDim.x, ... and loads the next tile into a second
+// register set while the current one is processed.
+std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
   const int K = s.k, NT = K - RB, NR = 1 << RB;
+  uint64_t tile_mask = 0;
+  for (int b : s.tbits) tile_mask |= 1ull << b;
   const char* R = prec == 0 ? "double" : "float";
   const char* C = prec == 0 ? "double2" : "float2";
   auto gbit = [&](int mi, int slot) { return s.tbits[s.maps[(size_t)mi * K + slot]]; };
@@ -64,18 +68,31 @@ std::string stage_source(const Stage& s, int prec, int RB) {
     return x;
   };
   std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << NT) << ") void qr_stage(" << C
-    << "* __restrict__ psi, const double2* __restrict__ cf, uint64_t nontile) {\n";
+  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << NT) << ")"
+    << (persistent ? " __attribute__((amdgpu_waves_per_eu(2, 2)))" : "") << " void qr_stage(" << C
+    << "* __restrict__ psi, const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles) {\n";
   o << "  using R = " << R << ";\n";
   o << "  __shared__ " << C << " lds[" << (1 << K) << "];\n";
   o << "  const uint32_t tid = threadIdx.x;\n";
-  o << "  uint64_t base = 0;\n";
-  o << "  { uint64_t m = nontile, t = blockIdx.x;\n"
-       "    while (m) { const uint64_t low = m & (0 - m); if (t & 1) base |= low; t >>= 1; m ^= low; } }\n";
+  o << "  uint64_t tile = blockIdx.x;\n";
+  o << "  if (tile >= ntiles) return;\n";
+  o << "  uint64_t base = qr::dev::deposit_bits(tile, nontile);\n";
   o << "  qr::dev::Regs<R, " << RB << "> r;\n";
   int mi = 0;
+  const std::string thread0 = thread_part(0).substr(4);  // " | ..." part after "base"
   o << "  uint64_t g0 = " << thread_part(mi) << ";\n";
   for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "] = psi[g0 | " << hex64(reg_off(mi, m)) << "];\n";
+  if (persistent) {
+    o << "  qr::dev::Regs<R, " << RB << "> nx;\n";
+    o << "  while (true) {\n";
+    o << "  const uint64_t ntile = tile + gridDim.x;\n";
+    o << "  const bool more = ntile < ntiles;\n";
+    o << "  uint64_t nbase = 0;\n";
+    o << "  if (more) {\n    nbase = qr::dev::deposit_bits(ntile, nontile);\n";
+    o << "    const uint64_t ng = nbase" << thread0 << ";\n";
+    for (int m = 0; m < NR; ++m) o << "    nx.v[" << m << "] = psi[ng | " << hex64(reg_off(0, m)) << "];\n";
+    o << "  }\n";
+  }
   auto coef = [&](size_t i) {
     std::ostringstream c;
     c << "qr::dev::Coef4{{cf[" << 4 * i << "], cf[" << 4 * i + 1 << "], cf[" << 4 * i + 2 << "], cf["
@@ -88,7 +105,10 @@ std::string stage_source(const Stage& s, int prec, int RB) {
     const char* has_ns = op.ns ? "true" : "false";
     switch (op.code) {
       case TC_U1: {
-        if (op.lane_cmask == 0) {
+        const uint64_t outside = op.lane_cmask & ~tile_mask, inside = op.lane_cmask & tile_mask;
+        if (inside == 0) {
+          // controls outside the tile are the same for the whole workgroup: a scalar branch
+          if (outside) o << "  if ((base & " << hex64(outside) << ") == " << hex64(outside) << ")";
           o << "  qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(r, "
             << coef(i) << ", " << ns << ");\n";
         } else {
@@ -103,7 +123,9 @@ std::string stage_source(const Stage& s, int prec, int RB) {
       case TC_D1: {
         const bool only1 = op.flags & 1;
         const int form = (int)(op.flags >> 4);
-        if (op.b != 0xFFu && op.lane_cmask == 0) {
+        const uint64_t outside = op.lane_cmask & ~tile_mask, inside = op.lane_cmask & tile_mask;
+        if (op.b != 0xFFu && inside == 0) {
+          if (outside) o << "  if ((base & " << hex64(outside) << ") == " << hex64(outside) << ")";
           o << "  qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", "
             << (only1 ? form : 0) << ", " << has_ns << ">(r, " << coef(i) << ", " << ns << ");\n";
         } else if (op.b != 0xFFu) {
@@ -153,6 +175,13 @@ std::string stage_source(const Stage& s, int prec, int RB) {
     }
   }
   for (int m = 0; m < NR; ++m) o << "  psi[g0 | " << hex64(reg_off(mi, m)) << "] = r.v[" << m << "];\n";
+  if (persistent) {
+    o << "  if (!more) break;\n";
+    o << "  tile = ntile;\n  base = nbase;\n";
+    o << "  g0 = " << thread_part(0) << ";\n";
+    for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "] = nx.v[" << m << "];\n";
+    o << "  }\n";
+  }
   o << "}\n";
   return o.str();
 }
@@ -194,10 +223,10 @@ bool compile_one(const std::string& src, std::vector<char>& code, std::string& e
 
 }  // namespace
 
-std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB) {
+std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB, bool persistent) {
   std::vector<std::string> out(stages.size());
   for (size_t i = 0; i < stages.size(); ++i)
-    if (stages[i].tile) out[i] = stage_source(stages[i], prec, RB);
+    if (stages[i].tile) out[i] = stage_source(stages[i], prec, RB, persistent);
   return out;
 }
 
@@ -241,8 +270,8 @@ void jit_prepare(const std::vector<std::string>& sources) {
   }
 }
 
-hipError_t jit_launch(const std::string& source, int K, int RB, void* psi, const double2* coefs, uint64_t ntiles,
-                      uint64_t nontile_mask, hipStream_t s) {
+hipError_t jit_launch(const std::string& source, int K, int RB, bool persistent, void* psi, const double2* coefs,
+                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s) {
   hipFunction_t fn = nullptr;
   {
     std::lock_guard<std::mutex> lk(g_mu);
@@ -251,8 +280,19 @@ hipError_t jit_launch(const std::string& source, int K, int RB, void* psi, const
     fn = it->second.fn;
   }
   if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
-  void* args[] = {&psi, (void*)&coefs, &nontile_mask};
-  return hipModuleLaunchKernel(fn, (unsigned)ntiles, 1, 1, 1u << (K - RB), 1, 1, 0, s, args, nullptr);
+  unsigned grid = (unsigned)ntiles;
+  const unsigned threads = 1u << (K - RB);
+  if (persistent) {
+    int dev = 0, cus = 0, per_cu = 0;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, 0) != hipSuccess || per_cu <= 0)
+      per_cu = 1;
+    if (cus <= 0) cus = 256;
+    grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
+  }
+  void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles};
+  return hipModuleLaunchKernel(fn, grid, 1, 1, threads, 1, 1, 0, s, args, nullptr);
 }
 
 size_t jit_cache_size() {

@@ -861,6 +861,15 @@ hipError_t launch_tile_t(bool prefetch, void* psi, const TileArgs& a, hipStream_
 
 template <typename R>
 hipError_t launch_tile_r(int K, int RB, bool prefetch, void* psi, const TileArgs& a, hipStream_t s) {
+  if (RB == 3 && K >= 10) {
+    switch (K) {
+      case 10: return launch_tile_t<R, 10, 3>(prefetch, psi, a, s);
+      case 11: return launch_tile_t<R, 11, 3>(prefetch, psi, a, s);
+      case 12: return launch_tile_t<R, 12, 3>(prefetch, psi, a, s);
+      case 13: return launch_tile_t<R, 13, 3>(prefetch, psi, a, s);
+      default: return hipErrorInvalidValue;
+    }
+  }
   if (RB == 5) {
     switch (K) {
       case 10: return launch_tile_t<R, 10, 5>(prefetch, psi, a, s);

@@ -77,6 +77,48 @@ int count_absorb(const std::vector<int>& remaining, const std::vector<Masks>& mk
   return c;
 }
 
+// Score of a register set for one phase: ops absorbed (x4), plus one for each absorbed
+// controlled op whose in-tile controls are all register bits (those controls then cost
+// nothing; thread-held ones cost a per-lane select of the whole 2x2).
+int phase_score(const std::vector<int>& remaining, const std::vector<Masks>& mk, const std::vector<uint8_t>& eligible,
+                const std::vector<uint64_t>& cmask, uint64_t regs, uint64_t tile) {
+  uint64_t bT = 0, bR = 0;
+  int score = 0;
+  for (int i : remaining) {
+    const uint64_t t = mk[i].t, r = mk[i].r;
+    const bool dep = (t & (bT | bR)) || (r & bT);
+    if (dep || !eligible[i] || (t & ~regs)) {
+      bT |= t;
+      bR |= r;
+    } else {
+      score += 4;
+      const uint64_t c = cmask[i] & tile;
+      if (c && (c & ~regs) == 0) score += 1;
+    }
+  }
+  return score;
+}
+
+uint64_t greedy_phase_bits(int want, const std::vector<int>& cands, const std::vector<int>& remaining,
+                           const std::vector<Masks>& mk, const std::vector<uint8_t>& eligible,
+                           const std::vector<uint64_t>& cmask, uint64_t tile) {
+  uint64_t cur = 0;
+  for (int step = 0; step < want; ++step) {
+    int best = -1, best_bit = -1;
+    for (int b : cands) {
+      if (cur & (1ull << b)) continue;
+      const int c = phase_score(remaining, mk, eligible, cmask, cur | (1ull << b), tile);
+      if (c > best) {
+        best = c;
+        best_bit = b;
+      }
+    }
+    if (best_bit < 0) break;
+    cur |= 1ull << best_bit;
+  }
+  return cur;
+}
+
 // Greedy choice of `want` extra bits from `cands` on top of `base` maximising absorption.
 uint64_t greedy_bits(uint64_t base, int want, const std::vector<int>& cands,
                      const std::vector<int>& remaining, const std::vector<Masks>& mk,
@@ -164,6 +206,8 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
   std::vector<Masks> mk(N);
   std::vector<uint8_t> eligible(N);
   const int k = std::min<int>(pp.tile_bits, (int)n);
+  std::vector<uint64_t> cmask(N);
+  for (int i = 0; i < N; ++i) cmask[i] = ops[i].cmask;
   for (int i = 0; i < N; ++i) {
     mk[i] = {ops[i].target_mask(), ops[i].read_mask()};
     eligible[i] = tile_eligible(ops[i], k) ? 1 : 0;
@@ -239,7 +283,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
 
     std::vector<int> rem = taken;
     // first register set: from the non-low tile bits so the load stays coalesced
-    uint64_t R = greedy_bits(0, r, nonlow_bits, rem, mk, eligible);
+    uint64_t R = greedy_phase_bits(r, nonlow_bits, rem, mk, eligible, cmask, B);
     for (int b : nonlow_bits)
       if (__builtin_popcountll(R) < r) R |= 1ull << b;
     map = make_map(R);
@@ -347,7 +391,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       rem = later;
       if (rem.empty() || hit_swap) continue;
       // next register set: any tile bits
-      uint64_t R2 = greedy_bits(0, r, tile_bits_vec, rem, mk, eligible);
+      uint64_t R2 = greedy_phase_bits(r, tile_bits_vec, rem, mk, eligible, cmask, B);
       for (int b : tile_bits_vec)
         if (__builtin_popcountll(R2) < r) R2 |= 1ull << b;
       if (now.empty() && count_absorb(rem, mk, eligible, R2) == 0)

@@ -163,5 +163,17 @@ __device__ __forceinline__ void scale_all(Regs<R, RB>& a, R fr, R fi, uint32_t n
 }
 
 
+// Deposits the bits of `v` into the set bits of `mask`, lowest first (uniform scalar work).
+__device__ __forceinline__ unsigned long long deposit_bits(unsigned long long v, unsigned long long mask) {
+  unsigned long long out = 0;
+  while (mask) {
+    const unsigned long long low = mask & (0ull - mask);
+    if (v & 1) out |= low;
+    v >>= 1;
+    mask ^= low;
+  }
+  return out;
+}
+
 }  // namespace dev
 }  // namespace qr

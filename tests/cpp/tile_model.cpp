@@ -147,7 +147,7 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
     PlanParams pp;
     pp.tile_bits = std::min<int>(tile_bits, (int)n);
     pp.reg_bits = pp.tile_bits < 4 ? pp.tile_bits : 4;
-    if (reg_bits == 5 && pp.tile_bits >= 10) pp.reg_bits = 5;
+    if ((reg_bits == 5 || reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = reg_bits;
     pp.low_bits = low_bits;
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);

@@ -82,13 +82,14 @@ def test_model_swap_heavy():
         assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12, trial
 
 
+@pytest.mark.parametrize("reg_bits", [3, 5])
 @pytest.mark.parametrize("tile_bits", [10, 11, 12, 13])
-def test_model_five_register_bits(tile_bits):
+def test_model_register_bits(tile_bits, reg_bits):
     rng = np.random.default_rng(2000 + tile_bits)
     for trial in range(4):
         n = int(rng.integers(tile_bits, 15))
         gates, customs = H.random_gates(n, 60, rng, with_custom=(trial % 2 == 0))
         p = H.to_program(gates, customs)
-        got, _ = run_model(n, p, tile_bits, low_bits=3, reg_bits=5)
+        got, _ = run_model(n, p, tile_bits, low_bits=3, reg_bits=reg_bits)
         want = H.oracle_simulate(n, p)
         assert np.max(np.abs(got - want)) < 1e-12, (trial, n)
