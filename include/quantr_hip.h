@@ -221,6 +221,11 @@ qr_status qr_grad_forward(qr_grad* grad, const qr_program* program,
                           const qr_observable* observable, double* expval_out);
 qr_status qr_grad_num_params(const qr_grad* grad, uint64_t* out);
 qr_status qr_grad_backward(qr_grad* grad, double grad_out, double* grads_out, uint64_t nparams);
+/* Engine options of both states of the context (as qr_state_set_options). With fused != 0
+ * and jit != 0 (the default) the reverse sweep runs as fused tile stages carrying psi and
+ * lam together, with gradient taps evaluated inside the tiles; otherwise one launch per
+ * op and state. */
+qr_status qr_grad_set_options(qr_grad* grad, const qr_options* options);
 
 /* ---- batched serving (no reference counterpart) ---------------------------------------- */
 

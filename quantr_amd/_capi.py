@@ -85,6 +85,7 @@ _SIGNATURES = [
                                   C.POINTER(C.c_double)]),
     ("qr_grad_num_params", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qr_grad_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_uint64]),
+    ("qr_grad_set_options", C.c_int, [C.c_void_p, C.c_void_p]),
     ("qr_batch_grad", C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("qr_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
@@ -330,6 +331,10 @@ class Grad:
         v = C.c_double()
         check(lib().qr_grad_forward(self._h, C.byref(p), C.byref(o), C.byref(v)))
         return v.value
+
+    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit)
+        check(lib().qr_grad_set_options(self._h, C.byref(o)))
 
     def num_params(self):
         v = C.c_uint64()

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -88,7 +89,6 @@ struct qr_state {
   hipStream_t stream = nullptr;
   int device = 0;
   qr_options opt{1, 0, 0, 2, 0, 0, 2};
-  std::vector<size_t> seen_structures;  // hashes of stage-structure lists run by the interpreter
   DevBuf aux;   // plan / coefficients / small uploads
   DevBuf aux2;  // partial sums, sampler scratch
   DevBuf tmp;   // wide-gate staging (bounded, never state-sized)
@@ -168,25 +168,18 @@ PlanParams plan_params(const qr_state* st) {
   return pp;
 }
 
-// Packs all stages and the standalone ops' tables into one device upload, then launches.
-void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_stats* stats) {
-  const uint32_t n = st->n;
-  const int B = (int)st->bytes_per_amp();
+// Device copy of a plan: every tile stage's ops, coefficients, slot maps and LDS patterns,
+// and the standalone ops' tables, in one upload.
+struct Loc {
+  size_t ops = 0, maps = 0, coefs = 0, off = 0;
+};
+struct PackedPlan {
   std::vector<Stage> stages;
-  const PlanParams pp = plan_params(st);
-  if (fused) {
-    stages = plan_fused(n, ops, pp, B, false);
-  } else {
-    for (size_t i = 0; i < ops.size(); ++i) {
-      Stage s;
-      s.tile = false;
-      s.standalone = (int)i;
-      s.bytes = 2.0 * (double)st->dim * B * ops[i].f_touch;
-      s.ngates = 1;
-      stages.push_back(std::move(s));
-    }
-  }
-  // ---- pack -------------------------------------------------------------------------
+  std::vector<Loc> loc;
+  uint8_t* dblob = nullptr;
+};
+
+void pack_plan(qr_state* st, const std::vector<FastOp>& ops, PackedPlan& P) {
   std::vector<uint8_t> blob;
   auto align = [&](size_t a) {
     while (blob.size() % a) blob.push_back(0);
@@ -197,10 +190,8 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
     if (bytes) std::memcpy(blob.data() + off, p, bytes);
     return off;
   };
-  struct Loc {
-    size_t ops = 0, maps = 0, coefs = 0, off = 0;
-  };
-  std::vector<Loc> loc(stages.size());
+  std::vector<Stage>& stages = P.stages;
+  P.loc.assign(stages.size(), Loc{});
   for (size_t si = 0; si < stages.size(); ++si) {
     Stage& s = stages[si];
     if (s.tile) {
@@ -217,56 +208,196 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
         t.map = maps_op ? h.a : 0;
         t.swz = h.swz;
         d[i] = t;
-        for (int q = 0; q < 4; ++q) c[4 * i + q] = make_double2(h.coef[q].real(), h.coef[q].imag());
+        if (h.src >= 0) {  // the current program's coefficients (plans are shared by structure)
+          const std::vector<cd>& m = ops[h.src].m;
+          for (int q = 0; q < 4; ++q)
+            c[4 * i + q] = q < (int)m.size() ? make_double2(m[q].real(), m[q].imag()) : make_double2(0.0, 0.0);
+        } else {
+          for (int q = 0; q < 4; ++q) c[4 * i + q] = make_double2(h.coef[q].real(), h.coef[q].imag());
+        }
       }
-      loc[si].ops = append(d.data(), d.size() * sizeof(TileOpDev));
+      P.loc[si].ops = append(d.data(), d.size() * sizeof(TileOpDev));
       align(16);
-      loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
+      P.loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
       align(4);
       std::vector<uint32_t> m(s.maps.size());
       for (size_t i = 0; i < m.size(); ++i)
         m[i] = (uint32_t)s.tbits[s.maps[i]] | ((uint32_t)s.maps[i] << 8);
-      loc[si].maps = append(m.data(), m.size() * sizeof(uint32_t));
+      P.loc[si].maps = append(m.data(), m.size() * sizeof(uint32_t));
       std::vector<uint32_t> z(s.swz.begin(), s.swz.end());
       if (z.empty()) z.push_back(0);
-      loc[si].off = append(z.data(), z.size() * sizeof(uint32_t));
+      P.loc[si].off = append(z.data(), z.size() * sizeof(uint32_t));
     } else {
       const FastOp& op = ops[s.standalone];
       if (op.kind == FK_DENSEK || op.kind == FK_DIAGK) {
         align(16);
         std::vector<double2> c(op.m.size());
         for (size_t i = 0; i < c.size(); ++i) c[i] = make_double2(op.m[i].real(), op.m[i].imag());
-        loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
+        P.loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
         align(8);
         const int dim = 1 << op.k;
         std::vector<uint64_t> off(dim, 0);
         for (int j = 0; j < dim; ++j)
           for (int q = 0; q < op.k; ++q)
             if ((j >> (op.k - 1 - q)) & 1) off[j] |= 1ull << op.bits[q];
-        loc[si].off = append(off.data(), off.size() * sizeof(uint64_t));
+        P.loc[si].off = append(off.data(), off.size() * sizeof(uint64_t));
       }
     }
   }
-  uint8_t* dblob = upload(st, st->aux, blob);
-  // ---- runtime-specialised stages --------------------------------------------------------
-  std::vector<std::string> jit_src;
-  bool use_jit = false;
+  P.dblob = upload(st, st->aux, blob);
+}
+
+TileArgs tile_args(const qr_state* st, const PackedPlan& P, size_t si) {
+  const Stage& s = P.stages[si];
+  TileArgs a{};
+  a.ops = (const TileOpDev*)(P.dblob + P.loc[si].ops);
+  a.coefs = (const double2*)(P.dblob + P.loc[si].coefs);
+  a.maps = (const uint32_t*)(P.dblob + P.loc[si].maps);
+  a.swz = (const uint32_t*)(P.dblob + P.loc[si].off);
+  a.nops = (uint32_t)s.ops.size();
+  a.ntiles = 1ull << (st->n - s.k);
+  uint64_t tm = 0;
+  for (int b : s.tbits) tm |= 1ull << b;
+  a.nontile_mask = (st->n >= 64 ? ~0ull : ((1ull << st->n) - 1)) & ~tm;
+  return a;
+}
+
+// One standalone (unfused) op on `amp`, a state of st's size, on st's stream.
+void launch_standalone(qr_state* st, void* amp, const FastOp& op, const PackedPlan& P, size_t si) {
+  const Stage& s = P.stages[si];
+  switch (op.kind) {
+    case FK_DENSE1: {
+      double2 u[4];
+      for (int i = 0; i < 4; ++i) u[i] = make_double2(op.m[i].real(), op.m[i].imag());
+      ProfScope ps_(st, "dense1", s.bytes);
+      hip_check(launch_dense1(st->prec, amp, st->dim / 2, op.bits[0], op.cmask, u, st->stream), "dense1");
+      break;
+    }
+    case FK_DIAG1: {
+      ProfScope ps_(st, "diag1", s.bytes);
+      hip_check(launch_diag1(st->prec, amp, st->dim, op.bits[0], op.cmask, make_double2(op.m[0].real(), op.m[0].imag()),
+                             make_double2(op.m[1].real(), op.m[1].imag()), st->stream),
+                "diag1");
+      break;
+    }
+    case FK_DENSEK: {
+      DenseKArgs a{};
+      a.k = op.k;
+      a.cmask = op.cmask;
+      std::vector<int> sb(op.bits, op.bits + op.k);
+      std::sort(sb.begin(), sb.end());
+      for (int q = 0; q < op.k; ++q) a.sorted_bits[q] = (uint8_t)sb[q];
+      const uint64_t dimk = 1ull << op.k;
+      const uint64_t ngroups = st->dim >> op.k;
+      uint64_t tg = std::min<uint64_t>(ngroups, std::max<uint64_t>(256, (64ull << 20) / (dimk * 16)));
+      tg = std::max<uint64_t>(tg / 256, 1) * 256;
+      if (ngroups < 256) tg = ngroups;
+      st->tmp.reserve(tg * dimk * st->bytes_per_amp());
+      ProfScope ps_(st, "densek", s.bytes);
+      hip_check(launch_densek(st->prec, amp, st->tmp.p, tg, ngroups, a, (const double2*)(P.dblob + P.loc[si].coefs),
+                              (const uint64_t*)(P.dblob + P.loc[si].off), st->stream),
+                "densek");
+      break;
+    }
+    case FK_DIAGK: {
+      DiagKArgs a{};
+      a.k = op.k;
+      a.cmask = op.cmask;
+      for (int q = 0; q < op.k; ++q) a.bits[q] = (uint8_t)op.bits[q];
+      ProfScope ps_(st, "diagk", s.bytes);
+      hip_check(launch_diagk(st->prec, amp, st->dim, a, (const double2*)(P.dblob + P.loc[si].coefs), st->stream),
+                "diagk");
+      break;
+    }
+    default:
+      throw Error{QR_ERR_UNSUPPORTED, "internal error: op kind has no standalone kernel"};
+  }
+}
+
+// ---- plan cache ---------------------------------------------------------------------------
+// Plans depend on the program's structure only (plan_key_append); parameter sweeps, gradient
+// loops and serving re-apply one structure with new coefficients, so plans and their
+// compiled stages are kept per process (least recently used dropped beyond 64).
+struct CachedPlan {
+  std::vector<Stage> stages;
+  std::vector<std::string> src;    // compiled-stage sources (filled on first compiled use)
+  std::vector<hipFunction_t> fns;  // their kernels
+  uint64_t runs = 0;
+  uint64_t stamp = 0;
+};
+std::mutex g_plan_mu;
+std::map<std::vector<uint64_t>, std::shared_ptr<CachedPlan>> g_plans;
+uint64_t g_plan_clock = 0;
+
+std::shared_ptr<CachedPlan> get_plan(const qr_state* st, const std::vector<FastOp>& ops, const PlanParams& pp,
+                                     bool fused, bool dual, bool persistent) {
+  std::vector<uint64_t> key{st->n, (uint64_t)st->prec, (uint64_t)pp.tile_bits, (uint64_t)pp.reg_bits,
+                            (uint64_t)pp.low_bits, (uint64_t)fused, (uint64_t)dual, (uint64_t)persistent};
+  key.reserve(8 + 3 * ops.size());
+  for (const FastOp& op : ops) plan_key_append(op, key);
+  {
+    std::lock_guard<std::mutex> lk(g_plan_mu);
+    auto it = g_plans.find(key);
+    if (it != g_plans.end()) {
+      it->second->stamp = ++g_plan_clock;
+      return it->second;
+    }
+  }
+  auto cp = std::make_shared<CachedPlan>();
+  const int B = (int)st->bytes_per_amp();
+  if (fused) {
+    cp->stages = plan_fused(st->n, ops, pp, B, dual);
+  } else {
+    for (size_t i = 0; i < ops.size(); ++i) {
+      Stage s;
+      s.tile = false;
+      s.standalone = (int)i;
+      s.bytes = 2.0 * (double)st->dim * B * ops[i].f_touch;
+      s.ngates = 1;
+      cp->stages.push_back(std::move(s));
+    }
+  }
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  cp->stamp = ++g_plan_clock;
+  g_plans[key] = cp;
+  while (g_plans.size() > 64) {
+    auto victim = g_plans.begin();
+    for (auto it = g_plans.begin(); it != g_plans.end(); ++it)
+      if (it->second->stamp < victim->second->stamp) victim = it;
+    g_plans.erase(victim);
+  }
+  return cp;
+}
+
+// Compiles (once) the tile stages of a cached plan and resolves their kernels.
+void prepare_compiled(CachedPlan& cp, int prec, int RB, bool persistent, bool dual) {
+  std::lock_guard<std::mutex> lk(g_plan_mu);
+  if (!cp.fns.empty() || cp.stages.empty()) return;
+  cp.src = jit_stage_sources(cp.stages, prec, RB, persistent, dual);
+  jit_prepare(cp.src);
+  std::vector<hipFunction_t> f(cp.stages.size(), nullptr);
+  for (size_t i = 0; i < cp.stages.size(); ++i)
+    if (cp.stages[i].tile) f[i] = jit_function(cp.src[i]);
+  cp.fns = std::move(f);
+}
+
+// Plans (fused tile sweeps, or one sweep per op), uploads and launches a fast-form program.
+void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_stats* stats) {
+  const PlanParams pp = plan_params(st);
   const bool persistent = st->opt.tile_variant == 2;
+  std::shared_ptr<CachedPlan> cp = get_plan(st, ops, pp, fused, false, persistent);
+  PackedPlan P;
+  P.stages = cp->stages;
+  std::vector<Stage>& stages = P.stages;
+  pack_plan(st, ops, P);
+  // ---- runtime-specialised stages: always with jit = 1, from 28 qubits or from the second
+  // application of a structure with jit = 2
+  bool use_jit = false;
   if (fused && st->opt.jit != 0) {
-    jit_src = jit_stage_sources(stages, st->prec, pp.reg_bits, persistent);
-    size_t h = 0;
-    for (const auto& x : jit_src) h = h * 1000003u ^ std::hash<std::string>()(x);
-    if (st->opt.jit == 1 || st->n >= 28) {
-      use_jit = true;
-    } else {
-      use_jit = std::find(st->seen_structures.begin(), st->seen_structures.end(), h) != st->seen_structures.end();
-      if (!use_jit) {
-        st->seen_structures.push_back(h);
-        if (st->seen_structures.size() > 64) st->seen_structures.erase(st->seen_structures.begin());
-      }
-    }
-    if (use_jit) jit_prepare(jit_src);
+    use_jit = st->opt.jit == 1 || st->n >= 28 || cp->runs > 0;
+    if (use_jit) prepare_compiled(*cp, st->prec, pp.reg_bits, persistent, false);
   }
+  cp->runs++;
   // ---- launch --------------------------------------------------------------------------
   qr_plan_stats ps{};
   for (size_t si = 0; si < stages.size(); ++si) {
@@ -275,19 +406,10 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
     ps.bytes += s.bytes;
     if (s.tile) {
       ps.tile_sweeps++;
-      TileArgs a{};
-      a.ops = (const TileOpDev*)(dblob + loc[si].ops);
-      a.coefs = (const double2*)(dblob + loc[si].coefs);
-      a.maps = (const uint32_t*)(dblob + loc[si].maps);
-      a.swz = (const uint32_t*)(dblob + loc[si].off);
-      a.nops = (uint32_t)s.ops.size();
-      a.ntiles = 1ull << (n - s.k);
-      uint64_t tm = 0;
-      for (int b : s.tbits) tm |= 1ull << b;
-      a.nontile_mask = (n >= 64 ? ~0ull : ((1ull << n) - 1)) & ~tm;
+      const TileArgs a = tile_args(st, P, si);
       ProfScope ps_(st, "tile_sweep", s.bytes);
       if (use_jit)
-        hip_check(jit_launch(jit_src[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles, a.nontile_mask,
+        hip_check(jit_launch(cp->fns[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles, a.nontile_mask,
                              st->stream),
                   "compiled tile kernel");
       else
@@ -295,55 +417,7 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
                   "tile kernel");
       continue;
     }
-    const FastOp& op = ops[s.standalone];
-    switch (op.kind) {
-      case FK_DENSE1: {
-        double2 u[4];
-        for (int i = 0; i < 4; ++i) u[i] = make_double2(op.m[i].real(), op.m[i].imag());
-        ProfScope ps_(st, "dense1", s.bytes);
-        hip_check(launch_dense1(st->prec, st->amp, st->dim / 2, op.bits[0], op.cmask, u, st->stream), "dense1");
-        break;
-      }
-      case FK_DIAG1: {
-        ProfScope ps_(st, "diag1", s.bytes);
-        hip_check(launch_diag1(st->prec, st->amp, st->dim, op.bits[0], op.cmask,
-                               make_double2(op.m[0].real(), op.m[0].imag()),
-                               make_double2(op.m[1].real(), op.m[1].imag()), st->stream),
-                  "diag1");
-        break;
-      }
-      case FK_DENSEK: {
-        DenseKArgs a{};
-        a.k = op.k;
-        a.cmask = op.cmask;
-        std::vector<int> sb(op.bits, op.bits + op.k);
-        std::sort(sb.begin(), sb.end());
-        for (int q = 0; q < op.k; ++q) a.sorted_bits[q] = (uint8_t)sb[q];
-        const uint64_t dimk = 1ull << op.k;
-        const uint64_t ngroups = st->dim >> op.k;
-        uint64_t tg = std::min<uint64_t>(ngroups, std::max<uint64_t>(256, (64ull << 20) / (dimk * 16)));
-        tg = std::max<uint64_t>(tg / 256, 1) * 256;
-        if (ngroups < 256) tg = ngroups;
-        st->tmp.reserve(tg * dimk * st->bytes_per_amp());
-        ProfScope ps_(st, "densek", s.bytes);
-        hip_check(launch_densek(st->prec, st->amp, st->tmp.p, tg, ngroups, a,
-                                (const double2*)(dblob + loc[si].coefs),
-                                (const uint64_t*)(dblob + loc[si].off), st->stream),
-                  "densek");
-        break;
-      }
-      case FK_DIAGK: {
-        DiagKArgs a{};
-        a.k = op.k;
-        a.cmask = op.cmask;
-        for (int q = 0; q < op.k; ++q) a.bits[q] = (uint8_t)op.bits[q];
-        ProfScope ps_(st, "diagk", s.bytes);
-        hip_check(launch_diagk(st->prec, st->amp, st->dim, a, (const double2*)(dblob + loc[si].coefs),
-                               st->stream),
-                  "diagk");
-        break;
-      }
-    }
+    launch_standalone(st, st->amp, ops[s.standalone], P, si);
   }
   ps.gates = 0;
   for (const Stage& s : stages) ps.gates += s.ngates;
@@ -872,12 +946,85 @@ qr_status qr_grad_forward(qr_grad* g, const qr_program* program, const qr_observ
   });
 }
 
+qr_status qr_grad_set_options(qr_grad* g, const qr_options* o) {
+  return guard([&] {
+    if (!g || !o) throw Error{QR_ERR_ARG, "NULL argument"};
+    qr_status st = qr_state_set_options(g->psi.get(), o);
+    if (st != QR_OK) throw Error{st, qr_last_error()};
+    st = qr_state_set_options(g->lam.get(), o);
+    if (st != QR_OK) throw Error{st, qr_last_error()};
+  });
+}
+
 qr_status qr_grad_num_params(const qr_grad* g, uint64_t* out) {
   return guard([&] {
     if (!g || !out) throw Error{QR_ERR_ARG, "NULL argument"};
     *out = g->nparams;
   });
 }
+
+namespace {
+
+// Tile shape of the adjoint sweep: psi and lam share a workgroup, so tiles are half the
+// width of the forward sweep's (2^10 complex128 / 2^11 complex64 amplitudes by default).
+PlanParams dual_plan_params(const qr_state* st) {
+  PlanParams pp;
+  int k = st->prec == 0 ? 10 : 11;
+  if (st->opt.tile_bits) k = std::min<int>(tile_max_bits(st->prec), (int)st->opt.tile_bits);
+  pp.tile_bits = std::min<int>(k, (int)st->n);
+  pp.reg_bits = std::min(4, pp.tile_bits);
+  pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
+  return pp;
+}
+
+// The reverse sweep as one fused program over (psi, lam): for every op from last to first,
+// a gradient tap (parameter ops) then the op's adjoint on both states. Tile stages carry
+// both states and write per-wavefront tap partials; standalone ops run on each state.
+void backward_fused(qr_grad* g, double grad_out, double* grads_out) {
+  qr_state* ps = g->psi.get();
+  qr_state* ls = g->lam.get();
+  hipStream_t s = ps->stream;
+  const std::vector<FastOp> rev = adjoint_sweep_ops(g->ops);
+  const PlanParams pp = dual_plan_params(ps);
+  std::shared_ptr<CachedPlan> cp = get_plan(ps, rev, pp, true, true, false);
+  prepare_compiled(*cp, ps->prec, pp.reg_bits, false, true);
+  PackedPlan P;
+  P.stages = cp->stages;
+  pack_plan(ps, rev, P);
+  const uint64_t nslots = jit_dual_slots((int)ps->n, pp.tile_bits, pp.reg_bits);
+  const uint64_t np = g->nparams;
+  ls->aux2.reserve((std::max<uint64_t>(np, 1) * nslots + np + 1) * sizeof(double));
+  double* part = (double*)ls->aux2.p;
+  double* sums = part + std::max<uint64_t>(np, 1) * nslots;
+  for (size_t si = 0; si < P.stages.size(); ++si) {
+    const Stage& st = P.stages[si];
+    if (st.tile) {
+      const TileArgs a = tile_args(ps, P, si);
+      ProfScope ps_(ps, "adjoint_sweep", 2.0 * st.bytes);
+      hip_check(jit_launch_dual(cp->fns[si], st.k, pp.reg_bits, ps->amp, ls->amp, a.coefs, a.ntiles, a.nontile_mask, part,
+                                nslots, s),
+                "compiled adjoint stage");
+    } else {
+      const FastOp& op = rev[st.standalone];
+      if (op.kind == FK_GRAD) throw Error{QR_ERR_UNSUPPORTED, "internal error: gradient tap outside a tile"};
+      launch_standalone(ps, ps->amp, op, P, si);
+      launch_standalone(ps, ls->amp, op, P, si);
+    }
+  }
+  hip_check(launch_sum_rows(part, nslots, (uint32_t)np, sums, s), "gradient sums");
+  std::vector<double> h(np);
+  if (np) hip_check(hipMemcpyAsync(h.data(), sums, np * sizeof(double), hipMemcpyDeviceToHost, s), "D2H");
+  hip_check(hipStreamSynchronize(s), "sync");
+  g->have_forward = false;
+  for (const auto& op : g->ops) {
+    if (op.param < 0) continue;
+    // dE/dtheta = Im <lam|G|psi> for exp(-i theta G/2); -Im <lam|psi> for exp(i theta/2) I
+    const double d = (op.gen == 4) ? -h[op.param] : h[op.param];
+    grads_out[op.param] = grad_out * d;
+  }
+}
+
+}  // namespace
 
 qr_status qr_grad_backward(qr_grad* g, double grad_out, double* grads_out, uint64_t nparams) {
   return guard([&] {
@@ -905,10 +1052,14 @@ qr_status qr_grad_backward(qr_grad* g, double grad_out, double* grads_out, uint6
       first = false;
     }
     if (first) hip_check(hipMemsetAsync(ls->amp, 0, ls->dim * ls->bytes_per_amp(), s), "memset");
+    if (ps->opt.fused && ps->opt.jit) {
+      backward_fused(g, grad_out, grads_out);
+      return;
+    }
     const int nb = 256;
     ls->aux2.reserve(std::max<uint64_t>(g->nparams, 1) * nb * sizeof(double2));
     double2* parts = (double2*)ls->aux2.p;
-    // reverse sweep
+    // reverse sweep, one launch per op and state
     for (size_t i = g->ops.size(); i-- > 0;) {
       const FastOp& op = g->ops[i];
       if (op.param >= 0) {

@@ -44,6 +44,7 @@ enum FastKind : int {
   FK_DIAG1 = 1,   // diag(d0, d1) on one bit (+ controls)
   FK_DENSEK = 2,  // 2^k x 2^k on k target bits (+ controls)
   FK_DIAGK = 3,   // 2^k phases on k bits (+ controls)
+  FK_GRAD = 4,    // adjoint sweep only: Im <lam| G |psi> of parameter `param` (G by `gen`)
 };
 
 // Specialisation hints of a 2x2 (kernels pick a cheaper form when exact).
@@ -80,6 +81,9 @@ std::string gate_debug(const qr_op& op, const uint32_t* controls);
 
 // Adjoint of a fast op (for the reverse sweep).
 FastOp adjoint(const FastOp& op);
+// The reverse sweep of a parameterised program as one op list over (psi, lam): from the
+// last op to the first, a gradient tap (FK_GRAD) for each parameter op, then the op's adjoint.
+std::vector<FastOp> adjoint_sweep_ops(const std::vector<FastOp>& ops);
 
 // ---- planning -------------------------------------------------------------------------------
 
@@ -91,6 +95,8 @@ struct TileOpHost {
   uint32_t ns = 0;       // register slots that must be 1 (controls held in registers)
   uint64_t lane_cmask = 0;  // control bits held by thread slots or outside the tile
   uint32_t swz = 0;      // REMAP: index of the LDS swizzle table
+  uint32_t param = 0;    // GRAD: parameter row of the partial-sum table
+  int src = -1;          // U1 / D1: index of the op whose matrix supplies the coefficients
   cd coef[4];
 };
 
@@ -114,5 +120,9 @@ struct PlanParams {
 
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
                               int bytes_per_amp, bool dual);
+// Appends what the planner's output depends on for one op (kind, bits, controls and the
+// value-derived forms), so that programs with equal keys share one plan; coefficients are
+// then taken from each program's own ops (TileOpHost::src).
+void plan_key_append(const FastOp& op, std::vector<uint64_t>& key);
 
 }  // namespace qr

@@ -47,13 +47,20 @@ std::string hex64(uint64_t v) {
 
 // Source of one stage kernel. `prec` 0 = complex128, 1 = complex64. With `persistent`, each
 // workgroup walks tiles blockIdx.x, +gridDim.x, ... and loads the next tile into a second
-// register set while the current one is processed.
-std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
-  const int K = s.k, NT = K - RB, NR = 1 << RB;
+// register set while the current one is processed. With `dual` (the adjoint sweep) the
+// kernel carries two states, psi and lam, through the same ops and evaluates gradient taps
+// (TC_GRAD) on them; each tap stores per-wavefront partials into its parameter's row.
+std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool dual) {
+  const int K = s.k, NT = K - RB, NR = 1 << RB, TPB = 1 << NT;
   uint64_t tile_mask = 0;
   for (int b : s.tbits) tile_mask |= 1ull << b;
   const char* R = prec == 0 ? "double" : "float";
   const char* C = prec == 0 ? "double2" : "float2";
+  const int amp_bytes = prec == 0 ? 16 : 8;
+  // both states through LDS at once when that fits in 64 KB, else one after the other
+  const bool lds2 = dual && (2 * (1 << K) * amp_bytes <= 65536);
+  std::vector<std::string> sets = dual ? std::vector<std::string>{"r", "l"} : std::vector<std::string>{"r"};
+  std::vector<std::string> arrays = dual ? std::vector<std::string>{"psi", "lam"} : std::vector<std::string>{"psi"};
   auto gbit = [&](int mi, int slot) { return s.tbits[s.maps[(size_t)mi * K + slot]]; };
   auto thread_part = [&](int mi) {
     std::ostringstream os;
@@ -68,20 +75,26 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
     return x;
   };
   std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(" << (1 << NT) << ")"
+  o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")"
     << (persistent ? " __attribute__((amdgpu_waves_per_eu(2, 2)))" : "") << " void qr_stage(" << C
-    << "* __restrict__ psi, const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles) {\n";
+    << "* __restrict__ psi, ";
+  if (dual) o << C << "* __restrict__ lam, ";
+  o << "const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles";
+  if (dual) o << ", double* __restrict__ part, uint64_t nslots";
+  o << ") {\n";
   o << "  using R = " << R << ";\n";
-  o << "  __shared__ " << C << " lds[" << (1 << K) << "];\n";
+  o << "  __shared__ " << C << " lds[" << (lds2 ? 2 : 1) * (1 << K) << "];\n";
   o << "  const uint32_t tid = threadIdx.x;\n";
   o << "  uint64_t tile = blockIdx.x;\n";
   o << "  if (tile >= ntiles) return;\n";
   o << "  uint64_t base = qr::dev::deposit_bits(tile, nontile);\n";
-  o << "  qr::dev::Regs<R, " << RB << "> r;\n";
+  for (const auto& v : sets) o << "  qr::dev::Regs<R, " << RB << "> " << v << ";\n";
   int mi = 0;
   const std::string thread0 = thread_part(0).substr(4);  // " | ..." part after "base"
   o << "  uint64_t g0 = " << thread_part(mi) << ";\n";
-  for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "] = psi[g0 | " << hex64(reg_off(mi, m)) << "];\n";
+  for (size_t v = 0; v < sets.size(); ++v)
+    for (int m = 0; m < NR; ++m)
+      o << "  " << sets[v] << ".v[" << m << "] = " << arrays[v] << "[g0 | " << hex64(reg_off(mi, m)) << "];\n";
   if (persistent) {
     o << "  qr::dev::Regs<R, " << RB << "> nx;\n";
     o << "  while (true) {\n";
@@ -109,14 +122,20 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
         if (inside == 0) {
           // controls outside the tile are the same for the whole workgroup: a scalar branch
           if (outside) o << "  if ((base & " << hex64(outside) << ") == " << hex64(outside) << ")";
-          o << "  qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(r, "
-            << coef(i) << ", " << ns << ");\n";
+          o << "  { const qr::dev::Coef4 u = " << coef(i) << ";";
+          for (const auto& v : sets)
+            o << " qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(" << v
+              << ", u, " << ns << ");";
+          o << " }\n";
         } else {
           o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
             << "    qr::dev::Coef4 u = " << coef(i) << ";\n"
             << "    u.c[0] = ok ? u.c[0] : make_double2(1.0, 0.0); u.c[1] = ok ? u.c[1] : make_double2(0.0, 0.0);\n"
-            << "    u.c[2] = ok ? u.c[2] : make_double2(0.0, 0.0); u.c[3] = ok ? u.c[3] : make_double2(1.0, 0.0);\n"
-            << "    qr::dev::u1_slot<R, " << RB << ", " << op.a << ", 0, " << has_ns << ">(r, u, " << ns << "); }\n";
+            << "    u.c[2] = ok ? u.c[2] : make_double2(0.0, 0.0); u.c[3] = ok ? u.c[3] : make_double2(1.0, 0.0);\n";
+          for (const auto& v : sets)
+            o << "    qr::dev::u1_slot<R, " << RB << ", " << op.a << ", 0, " << has_ns << ">(" << v << ", u, " << ns
+              << ");\n";
+          o << "  }\n";
         }
         break;
       }
@@ -126,22 +145,47 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
         const uint64_t outside = op.lane_cmask & ~tile_mask, inside = op.lane_cmask & tile_mask;
         if (op.b != 0xFFu && inside == 0) {
           if (outside) o << "  if ((base & " << hex64(outside) << ") == " << hex64(outside) << ")";
-          o << "  qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", "
-            << (only1 ? form : 0) << ", " << has_ns << ">(r, " << coef(i) << ", " << ns << ");\n";
+          o << "  { const qr::dev::Coef4 d = " << coef(i) << ";";
+          for (const auto& v : sets)
+            o << " qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", "
+              << (only1 ? form : 0) << ", " << has_ns << ">(" << v << ", d, " << ns << ");";
+          o << " }\n";
         } else if (op.b != 0xFFu) {
           o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
             << "    qr::dev::Coef4 d = " << coef(i) << ";\n"
-            << "    d.c[0] = ok ? d.c[0] : make_double2(1.0, 0.0); d.c[1] = ok ? d.c[1] : make_double2(1.0, 0.0);\n"
-            << "    qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", 0, "
-            << has_ns << ">(r, d, " << ns << "); }\n";
+            << "    d.c[0] = ok ? d.c[0] : make_double2(1.0, 0.0); d.c[1] = ok ? d.c[1] : make_double2(1.0, 0.0);\n";
+          for (const auto& v : sets)
+            o << "    qr::dev::d1_reg<R, " << RB << ", " << op.b << ", " << (only1 ? "true" : "false") << ", 0, "
+              << has_ns << ">(" << v << ", d, " << ns << ");\n";
+          o << "  }\n";
         } else {
           o << "  { const bool bit = (g0 >> " << op.a << ") & 1;\n"
             << "    double2 f = bit ? cf[" << 4 * i + 1 << "] : cf[" << 4 * i << "];\n";
           if (op.lane_cmask)
             o << "    const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
               << "    f = ok ? f : make_double2(1.0, 0.0);\n";
-          o << "    qr::dev::scale_all<R, " << RB << ", 0, " << has_ns << ">(r, (R)f.x, (R)f.y, " << ns << "); }\n";
+          for (const auto& v : sets)
+            o << "    qr::dev::scale_all<R, " << RB << ", 0, " << has_ns << ">(" << v << ", (R)f.x, (R)f.y, " << ns
+              << ");\n";
+          o << "  }\n";
         }
+        break;
+      }
+      case TC_GRAD: {
+        if (!dual) break;
+        const int gen = (int)op.flags;
+        o << "  { double acc;\n";
+        if (gen == 1 || gen == 2) {
+          o << "    acc = qr::dev::tap_pair<R, " << RB << ", " << op.a << ", " << gen << ">(l, r);\n";
+        } else if (gen == 3 && op.b != 0xFFu) {
+          o << "    acc = qr::dev::tap_diag<R, " << RB << ", " << op.b << ">(l, r);\n";
+        } else if (gen == 3) {
+          o << "    acc = qr::dev::tap_diag<R, " << RB << ", -1>(l, r);\n"
+            << "    if ((g0 >> " << op.a << ") & 1) acc = -acc;\n";
+        } else {
+          o << "    acc = qr::dev::tap_diag<R, " << RB << ", -1>(l, r);\n";
+        }
+        o << "    qr::dev::tap_store<" << TPB << ">(acc, part + (uint64_t)" << op.param << "u * nslots, tile, tid); }\n";
         break;
       }
       case TC_REMAP: {
@@ -158,10 +202,27 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
             if ((m >> j) & 1) x ^= Q[NT + j];
           return x;
         };
-        o << "  { __syncthreads();\n    const uint32_t w = " << lds_thread(P) << ";\n";
-        for (int m = 0; m < NR; ++m) o << "    lds[w ^ " << lds_reg(P, m) << "u] = r.v[" << m << "];\n";
-        o << "    __syncthreads();\n    const uint32_t q = " << lds_thread(P + K) << ";\n";
-        for (int m = 0; m < NR; ++m) o << "    r.v[" << m << "] = lds[q ^ " << lds_reg(P + K, m) << "u];\n";
+        o << "  { const uint32_t w = " << lds_thread(P) << ";\n";
+        o << "    const uint32_t q = " << lds_thread(P + K) << ";\n";
+        if (lds2 || !dual) {
+          o << "    __syncthreads();\n";
+          for (size_t v = 0; v < sets.size(); ++v)
+            for (int m = 0; m < NR; ++m)
+              o << "    lds[" << v * (1 << K) << "u + (w ^ " << lds_reg(P, m) << "u)] = " << sets[v] << ".v[" << m
+                << "];\n";
+          o << "    __syncthreads();\n";
+          for (size_t v = 0; v < sets.size(); ++v)
+            for (int m = 0; m < NR; ++m)
+              o << "    " << sets[v] << ".v[" << m << "] = lds[" << v * (1 << K) << "u + (q ^ " << lds_reg(P + K, m)
+                << "u)];\n";
+        } else {
+          for (const auto& v : sets) {
+            o << "    __syncthreads();\n";
+            for (int m = 0; m < NR; ++m) o << "    lds[w ^ " << lds_reg(P, m) << "u] = " << v << ".v[" << m << "];\n";
+            o << "    __syncthreads();\n";
+            for (int m = 0; m < NR; ++m) o << "    " << v << ".v[" << m << "] = lds[q ^ " << lds_reg(P + K, m) << "u];\n";
+          }
+        }
         mi = (int)op.a;
         o << "    g0 = " << thread_part(mi) << "; }\n";
         break;
@@ -174,7 +235,9 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent) {
         break;
     }
   }
-  for (int m = 0; m < NR; ++m) o << "  psi[g0 | " << hex64(reg_off(mi, m)) << "] = r.v[" << m << "];\n";
+  for (size_t v = 0; v < sets.size(); ++v)
+    for (int m = 0; m < NR; ++m)
+      o << "  " << arrays[v] << "[g0 | " << hex64(reg_off(mi, m)) << "] = " << sets[v] << ".v[" << m << "];\n";
   if (persistent) {
     o << "  if (!more) break;\n";
     o << "  tile = ntile;\n  base = nbase;\n";
@@ -223,10 +286,11 @@ bool compile_one(const std::string& src, std::vector<char>& code, std::string& e
 
 }  // namespace
 
-std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB, bool persistent) {
+std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB, bool persistent,
+                                           bool dual) {
   std::vector<std::string> out(stages.size());
   for (size_t i = 0; i < stages.size(); ++i)
-    if (stages[i].tile) out[i] = stage_source(stages[i], prec, RB, persistent);
+    if (stages[i].tile) out[i] = stage_source(stages[i], prec, RB, persistent && !dual, dual);
   return out;
 }
 
@@ -270,15 +334,16 @@ void jit_prepare(const std::vector<std::string>& sources) {
   }
 }
 
-hipError_t jit_launch(const std::string& source, int K, int RB, bool persistent, void* psi, const double2* coefs,
+hipFunction_t jit_function(const std::string& source) {
+  std::lock_guard<std::mutex> lk(g_mu);
+  auto it = g_cache.find(source);
+  if (it == g_cache.end()) throw Error{QR_ERR_HIP, "internal error: tile stage used before it was compiled"};
+  return it->second.fn;
+}
+
+hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* psi, const double2* coefs,
                       uint64_t ntiles, uint64_t nontile_mask, hipStream_t s) {
-  hipFunction_t fn = nullptr;
-  {
-    std::lock_guard<std::mutex> lk(g_mu);
-    auto it = g_cache.find(source);
-    if (it == g_cache.end()) return hipErrorNotFound;
-    fn = it->second.fn;
-  }
+  if (!fn) return hipErrorInvalidValue;
   if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
   unsigned grid = (unsigned)ntiles;
   const unsigned threads = 1u << (K - RB);
@@ -293,6 +358,22 @@ hipError_t jit_launch(const std::string& source, int K, int RB, bool persistent,
   }
   void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles};
   return hipModuleLaunchKernel(fn, grid, 1, 1, threads, 1, 1, 0, s, args, nullptr);
+}
+
+hipError_t jit_launch_dual(hipFunction_t fn, int K, int RB, void* psi, void* lam, const double2* coefs,
+                           uint64_t ntiles, uint64_t nontile_mask, double* part, uint64_t nslots, hipStream_t s) {
+  if (!fn) return hipErrorInvalidValue;
+  if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
+  const unsigned threads = 1u << (K - RB);
+  void* args[] = {&psi, &lam, (void*)&coefs, &nontile_mask, &ntiles, &part, &nslots};
+  return hipModuleLaunchKernel(fn, (unsigned)ntiles, 1, 1, threads, 1, 1, 0, s, args, nullptr);
+}
+
+// Partials written per tile by a dual stage: one per wavefront (or one per tile when a tile
+// has fewer than 64 threads).
+uint64_t jit_dual_slots(int n, int K, int RB) {
+  const uint64_t tpb = 1ull << (K - RB);
+  return (1ull << (n - K)) * (tpb < 64 ? 1 : tpb / 64);
 }
 
 size_t jit_cache_size() {

@@ -11,12 +11,20 @@
 namespace qr {
 
 // Kernel source of every tile stage (empty string for standalone stages).
-std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB, bool persistent);
+// With `dual`, stages carry psi and lam together and evaluate gradient taps (adjoint sweep).
+std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int prec, int RB, bool persistent,
+                                           bool dual = false);
 // Compiles (in parallel) and loads every source not yet in the process-wide cache. Throws Error.
 void jit_prepare(const std::vector<std::string>& sources);
+// The kernel of a prepared source. Throws Error if it was not prepared.
+hipFunction_t jit_function(const std::string& source);
 // Launches a prepared stage: one workgroup per tile.
-hipError_t jit_launch(const std::string& source, int K, int RB, bool persistent, void* psi, const double2* coefs,
+hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* psi, const double2* coefs,
                       uint64_t ntiles, uint64_t nontile_mask, hipStream_t s);
+// Launches a prepared dual stage; gradient taps write part[param * nslots + slot].
+hipError_t jit_launch_dual(hipFunction_t fn, int K, int RB, void* psi, void* lam, const double2* coefs,
+                           uint64_t ntiles, uint64_t nontile_mask, double* part, uint64_t nslots, hipStream_t s);
+uint64_t jit_dual_slots(int n, int K, int RB);
 size_t jit_cache_size();
 
 }  // namespace qr

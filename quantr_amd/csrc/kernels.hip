@@ -712,6 +712,23 @@ __global__ __launch_bounds__(256) void k_gen_partial(const typename CT<R>::T* __
   if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
 }
 
+// Row sums of a partial-sum table in a fixed order (one workgroup per row): each thread adds
+// a strided subset, then a tree over the workgroup.
+__global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ part, uint64_t ncols,
+                                                  double* __restrict__ out) {
+  __shared__ double red[256];
+  const double* row = part + (uint64_t)blockIdx.x * ncols;
+  double acc = 0.0;
+  for (uint64_t j = threadIdx.x; j < ncols; j += 256) acc += row[j];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = red[0];
+}
+
 // complex64 <-> binary64 conversion for state upload/download
 __global__ void k_f64_to_c64(float2* __restrict__ dst, const double2* __restrict__ src, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -977,6 +994,12 @@ hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64
     dev::k_gen_partial<double><<<nblocks, 256, 0, s>>>((const double2*)lam, (const double2*)psi, dim, t, gen, part);
   else
     dev::k_gen_partial<float><<<nblocks, 256, 0, s>>>((const float2*)lam, (const float2*)psi, dim, t, gen, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum_rows(const double* part, uint64_t ncols, uint32_t nrows, double* out, hipStream_t s) {
+  if (nrows == 0) return hipSuccess;
+  dev::k_sum_rows<<<nrows, 256, 0, s>>>(part, ncols, out);
   return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ enum TileCode : uint32_t {
   TC_U1 = 2,         // a = register slot, coef -> 4 complex, b = form
   TC_D1 = 3,         // a = global bit, b = register slot or 0xFF, coef -> 2 complex, c = only |1>
   TC_SWAPSLOTS = 4,  // a = index of the new slot map (relabel, no data movement)
+  TC_GRAD = 5,       // compiled dual-state stages only: gradient partial of one parameter
 };
 
 // One interpreted op of a tile program (32 bytes, read with scalar loads).
@@ -107,6 +108,8 @@ hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim
                               double cr, double ci, int accumulate, hipStream_t s);
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s);
+// out[r] = sum of part[r * ncols + 0 .. ncols-1] (fixed order), r < nrows.
+hipError_t launch_sum_rows(const double* part, uint64_t ncols, uint32_t nrows, double* out, hipStream_t s);
 hipError_t launch_f64_to_c64(float2* dst, const double2* src, uint64_t n, hipStream_t s);
 hipError_t launch_c64_to_f64(double2* dst, const float2* src, uint64_t n, hipStream_t s);
 

@@ -118,6 +118,9 @@ const char* last_error_cstr() { return t_err.c_str(); }
 
 uint64_t FastOp::target_mask() const {
   if (kind == FK_DIAG1 || kind == FK_DIAGK) return 0;
+  // a gradient tap reads psi and lam; X/Y generators pair amplitudes across their bit, so
+  // they are ordered like a gate on it; Z and the identity commute with every diagonal
+  if (kind == FK_GRAD) return (gen == 1 || gen == 2) ? (1ull << bits[0]) : 0;
   uint64_t m = 0;
   for (int i = 0; i < k; ++i) m |= 1ull << bits[i];
   return m;
@@ -125,6 +128,7 @@ uint64_t FastOp::target_mask() const {
 
 uint64_t FastOp::read_mask() const {
   uint64_t m = cmask;
+  if (kind == FK_GRAD) return gen == 3 ? (1ull << bits[0]) : 0;
   if (kind == FK_DIAG1 || kind == FK_DIAGK)
     for (int i = 0; i < k; ++i) m |= 1ull << bits[i];
   return m;
@@ -369,6 +373,26 @@ std::vector<FastOp> lower_fast(uint32_t n, const std::vector<ExactGate>& gates, 
     out.push_back(std::move(f));
   }
   return out;
+}
+
+std::vector<FastOp> adjoint_sweep_ops(const std::vector<FastOp>& ops) {
+  std::vector<FastOp> rev;
+  rev.reserve(2 * ops.size());
+  for (size_t i = ops.size(); i-- > 0;) {
+    const FastOp& op = ops[i];
+    if (op.param >= 0) {
+      FastOp t;
+      t.kind = FK_GRAD;
+      t.k = 1;
+      t.bits[0] = op.bits[0];
+      t.gen = op.gen;
+      t.param = op.param;
+      t.f_touch = 0.0;
+      rev.push_back(t);
+    }
+    rev.push_back(adjoint(op));
+  }
+  return rev;
 }
 
 FastOp adjoint(const FastOp& op) {

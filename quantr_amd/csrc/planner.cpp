@@ -34,7 +34,12 @@ bool is_swap_perm(const FastOp& op) {
 }
 
 bool tile_eligible(const FastOp& op, int k) {
-  return op.kind == FK_DENSE1 || op.kind == FK_DIAG1 || (k >= 2 && is_swap_perm(op));
+  return op.kind == FK_DENSE1 || op.kind == FK_DIAG1 || op.kind == FK_GRAD || (k >= 2 && is_swap_perm(op));
+}
+
+// Ops that need their target bit in a register slot (a 2x2 across it).
+bool needs_reg_target(const FastOp& op) {
+  return op.kind == FK_DENSE1 || (op.kind == FK_GRAD && (op.gen == 1 || op.gen == 2));
 }
 
 struct Masks {
@@ -199,6 +204,29 @@ std::vector<uint16_t> choose_swizzle(int k, int nts, const std::vector<int>& old
 
 }  // namespace
 
+void plan_key_append(const FastOp& op, std::vector<uint64_t>& key) {
+  uint64_t cls = 0;
+  if (op.kind == FK_DENSE1) {
+    cls = (uint64_t)op.form;
+    if (op.form == U1_PERM && op.m[1] == cd(1, 0) && op.m[2] == cd(1, 0)) cls = 3;
+  } else if (op.kind == FK_DIAG1) {
+    if (op.m[0] == cd(1, 0)) {
+      cls = 1;
+      if (op.m[1] == cd(-1, 0)) cls |= 1u << 4;
+      else if (op.m[1] == cd(0, 1)) cls |= 2u << 4;
+      else if (op.m[1] == cd(0, -1)) cls |= 3u << 4;
+    }
+  } else if (op.kind == FK_DENSEK) {
+    cls = is_swap_perm(op) ? 1 : 0;
+  }
+  uint64_t bits = 0;
+  for (int q = 0; q < op.k && q < 8; ++q) bits |= (uint64_t)(op.bits[q] & 0xFF) << (8 * q);
+  key.push_back((uint64_t)op.kind | ((uint64_t)op.k << 8) | ((uint64_t)op.gen << 16) | (cls << 24) |
+                ((uint64_t)(op.kind == FK_GRAD ? op.param : 0) << 32));
+  key.push_back(op.cmask);
+  key.push_back(bits);
+}
+
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
                               int bytes_per_amp, bool /*dual*/) {
   std::vector<Stage> stages;
@@ -339,7 +367,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           const uint64_t t = mk[i].t, rr = mk[i].r;
           const bool dep = (t & (bT | bR)) || (rr & bT);
           bool ok = !dep;
-          if (ok && op.kind == FK_DENSE1) ok = (t & ~regs) == 0;
+          if (ok && needs_reg_target(op)) ok = (t & ~regs) == 0;
           if (ok && op.kind == FK_DENSEK) ok = (t & ~B) == 0;
           if (ok) {
             now.push_back(i);
@@ -363,6 +391,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           int form = op.form;
           if (form == U1_PERM && op.m[1] == cd(1, 0) && op.m[2] == cd(1, 0)) form = 3;  // X
           t.b = (uint32_t)form;
+          t.src = i;
           s.ops.push_back(t);
         } else if (op.kind == FK_DIAG1) {
           t.code = TC_D1;
@@ -372,11 +401,25 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           t.b = (sl >= nts) ? (uint32_t)(sl - nts) : 0xFFu;
           t.coef[0] = op.m[0];
           t.coef[1] = op.m[1];
+          t.src = i;
           if (op.m[0] == cd(1, 0)) {
             t.flags = 1;  // only the |1> half changes
             if (op.m[1] == cd(-1, 0)) t.flags |= 1u << 4;
             else if (op.m[1] == cd(0, 1)) t.flags |= 2u << 4;
             else if (op.m[1] == cd(0, -1)) t.flags |= 3u << 4;
+          }
+          s.ops.push_back(t);
+        } else if (op.kind == FK_GRAD) {
+          t.code = TC_GRAD;
+          t.flags = (uint32_t)op.gen;
+          t.param = (uint32_t)op.param;
+          if (op.gen == 1 || op.gen == 2) {
+            t.a = (uint32_t)(slot_of_pos(pos_of_bit[op.bits[0]]) - nts);
+          } else {
+            t.a = (uint32_t)op.bits[0];
+            const int p = op.gen == 3 ? pos_of_bit[op.bits[0]] : -1;
+            const int sl = p >= 0 ? slot_of_pos(p) : -1;
+            t.b = (sl >= nts) ? (uint32_t)(sl - nts) : 0xFFu;
           }
           s.ops.push_back(t);
         } else {  // swap: exchange the tile positions held by the two slots

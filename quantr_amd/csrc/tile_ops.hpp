@@ -162,6 +162,54 @@ __device__ __forceinline__ void scale_all(Regs<R, RB>& a, R fr, R fi, uint32_t n
   }
 }
 
+// ---- adjoint sweep: gradient taps ----------------------------------------------------------
+// Im(conj(l) p) and Re(conj(l) p), accumulated in binary64 for both precisions.
+template <typename C>
+__device__ __forceinline__ double imc(const C& l, const C& p) {
+  return (double)l.x * (double)p.y - (double)l.y * (double)p.x;
+}
+template <typename C>
+__device__ __forceinline__ double rec(const C& l, const C& p) {
+  return (double)l.x * (double)p.x + (double)l.y * (double)p.y;
+}
+
+// sum_j Im(conj(l_j) (G p)_j) over this thread's registers, G = X (GEN 1) or Y (GEN 2) on
+// register slot J: (X p)_j = p_{j^J}; (Y p)_j = -i (-1)^{bit J of j} p_{j^J}.
+template <typename R, int RB, int J, int GEN>
+__device__ __forceinline__ double tap_pair(const Regs<R, RB>& l, const Regs<R, RB>& p) {
+  double acc = 0.0;
+#pragma unroll
+  for (int m0 = 0; m0 < (1 << RB); ++m0) {
+    if (m0 & (1 << J)) continue;
+    const int m1 = m0 | (1 << J);
+    if (GEN == 1) acc += imc(l.v[m0], p.v[m1]) + imc(l.v[m1], p.v[m0]);
+    else acc += rec(l.v[m1], p.v[m0]) - rec(l.v[m0], p.v[m1]);
+  }
+  return acc;
+}
+
+// sum_j s_j Im(conj(l_j) p_j) with s_j = (-1)^{bit J of j} (J >= 0) or 1 (J < 0).
+template <typename R, int RB, int J>
+__device__ __forceinline__ double tap_diag(const Regs<R, RB>& l, const Regs<R, RB>& p) {
+  double acc = 0.0;
+#pragma unroll
+  for (int m = 0; m < (1 << RB); ++m) {
+    const double v = imc(l.v[m], p.v[m]);
+    if (J >= 0 && (m & (1 << J))) acc -= v;
+    else acc += v;
+  }
+  return acc;
+}
+
+// Sums `acc` over each group of min(TPB, 64) threads (one wavefront, or the whole workgroup
+// when it is smaller) and stores one partial per group: row[tile * groups + group].
+template <int TPB>
+__device__ __forceinline__ void tap_store(double acc, double* row, unsigned long long tile, unsigned int tid) {
+  constexpr int W = TPB < 64 ? TPB : 64;
+#pragma unroll
+  for (int off = W / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, W);
+  if ((tid & (W - 1)) == 0) row[tile * (unsigned long long)(TPB / W) + tid / W] = acc;
+}
 
 // Deposits the bits of `v` into the set bits of `mask`, lowest first (uniform scalar work).
 __device__ __forceinline__ unsigned long long deposit_bits(unsigned long long v, unsigned long long mask) {

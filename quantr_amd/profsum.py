@@ -1,0 +1,80 @@
+"""Summarise rocprofv3 CSV output into the short text/JSON records kept under profiles/.
+
+Usage:
+    python -m quantr_amd.profsum stats  <run_kernel_stats.csv> <out.md> <command...>
+    python -m quantr_amd.profsum traffic <FETCH_SIZE csv> <WRITE_SIZE csv> <kernel> <bytes> <out> <command...>
+
+`traffic` scales both counters with kernels of known traffic that the same process ran:
+`k_norm_partial` reads the whole state once (FETCH_SIZE calibration) and `k_init_basis`
+writes it once (WRITE_SIZE calibration); the state size in bytes is the `bytes` argument
+(one read + one write of it is the tile sweep's algorithmic traffic).
+"""
+import collections
+import csv
+import json
+import sys
+
+
+def per_kernel(path, counter=None):
+    d = collections.defaultdict(list)
+    with open(path) as f:
+        for r in csv.DictReader(f):
+            if counter is not None and r["Counter_Name"] != counter:
+                continue
+            d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
+
+
+def find(d, sub):
+    for k in d:
+        if sub in k:
+            return k
+    raise KeyError(sub)
+
+
+def stats(csv_path, out, cmd):
+    rows = list(csv.DictReader(open(csv_path)))
+    lines = ["# rocprofv3 --kernel-trace --stats", "", "Command (from the repository root on the GPU host):", "",
+             "    " + cmd, "", "| kernel | calls | avg ms | min ms | max ms | share % |", "|---|---|---|---|---|---|"]
+    for r in rows:
+        name = r["Name"] if len(r["Name"]) < 70 else r["Name"][:67] + "..."
+        lines.append("| %s | %s | %.4f | %.4f | %.4f | %s |" % (
+            name, r["Calls"], float(r["AverageNs"]) / 1e6, float(r["MinNs"]) / 1e6,
+            float(r["MaxNs"]) / 1e6, r["Percentage"]))
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
+def traffic(fetch_csv, write_csv, kernel, state_bytes, out, cmd):
+    fd, fn = per_kernel(fetch_csv, "FETCH_SIZE")
+    wd, wn = per_kernel(write_csv, "WRITE_SIZE")
+    kib = state_bytes / 1024.0
+    f_scale = kib / fd[find(fd, "k_norm_partial")]
+    w_scale = kib / wd[find(wd, "k_init_basis")]
+    kf, kw = find(fd, kernel), find(wd, kernel)
+    fetch_b = fd[kf] * f_scale * 1024.0
+    write_b = wd[kw] * w_scale * 1024.0
+    rec = {
+        "kernel": kernel, "command": cmd, "dispatches": fn[kf],
+        "fetch_kib_raw": fd[kf], "write_kib_raw": wd[kw],
+        "fetch_scale": f_scale, "write_scale": w_scale,
+        "fetch_bytes_per_launch": fetch_b, "write_bytes_per_launch": write_b,
+        "traffic_bytes_per_launch": fetch_b + write_b,
+        "algorithmic_bytes_per_launch": 2.0 * state_bytes,
+        "traffic_over_algorithmic": (fetch_b + write_b) / (2.0 * state_bytes),
+        "calibration": "FETCH_SIZE scaled by k_norm_partial (reads the state once), "
+                       "WRITE_SIZE by k_init_basis (writes the state once), same process",
+    }
+    with open(out, "w") as f:
+        json.dump(rec, f, indent=1)
+        f.write("\n")
+    return rec
+
+
+if __name__ == "__main__":
+    if sys.argv[1] == "stats":
+        stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
+    elif sys.argv[1] == "traffic":
+        print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]), sys.argv[6],
+                                 " ".join(sys.argv[7:])), indent=1))
+    else:
+        raise SystemExit("usage: profsum stats|traffic ...")

@@ -51,9 +51,15 @@ static void apply_standalone(uint32_t n, std::vector<cdv>& psi, const FastOp& op
   }
 }
 
+static double imc(cdv l, cdv p) { return l.real() * p.imag() - l.imag() * p.real(); }
+static double rec(cdv l, cdv p) { return l.real() * p.real() + l.imag() * p.imag(); }
+
 // One tile stage, modelled thread by thread exactly as k_tile computes indices (maps with
 // global bits and tile positions, per-lane / register-slot control split, swizzled LDS).
-static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB) {
+// With `lam` (the adjoint sweep) both states go through every op and gradient taps add
+// sum_j Im(conj(lam_j) (G psi)_j) into grads[param].
+static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, std::vector<cdv>* lam = nullptr,
+                     std::vector<double>* grads = nullptr) {
   const int K = s.k, NT = K - RB;
   const uint64_t ntiles = 1ull << (n - K);
   uint64_t tm = 0;
@@ -77,30 +83,67 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB) 
         if ((m >> j) & 1) x |= 1ull << gbit(cur_map, NT + j);
       return x;
     };
-    std::vector<std::vector<cdv>> reg(T, std::vector<cdv>(R));
+    std::vector<std::vector<cdv>> reg(T, std::vector<cdv>(R)), lreg(T, std::vector<cdv>(R));
     for (int t = 0; t < T; ++t)
-      for (int m = 0; m < R; ++m) reg[t][m] = psi[gidx(t, m)];
+      for (int m = 0; m < R; ++m) {
+        reg[t][m] = psi[gidx(t, m)];
+        if (lam) lreg[t][m] = (*lam)[gidx(t, m)];
+      }
+    std::vector<std::vector<std::vector<cdv>>*> sets{&reg};
+    if (lam) sets.push_back(&lreg);
     for (const TileOpHost& op : s.ops) {
-      for (int t = 0; t < T && (op.code == TC_U1 || op.code == TC_D1); ++t) {
-        const uint64_t g0 = gidx(t, 0);
-        const bool lane_ok = (g0 & op.lane_cmask) == op.lane_cmask;
-        if (!lane_ok) continue;
-        if (op.code == TC_U1) {
-          const int J = (int)op.a;
-          for (int m0 = 0; m0 < R; ++m0) {
-            if (m0 & (1 << J)) continue;
-            if (((uint32_t)m0 & op.ns) != op.ns) continue;
-            const int m1 = m0 | (1 << J);
-            const cdv x = reg[t][m0], y = reg[t][m1];
-            reg[t][m0] = op.coef[0] * x + op.coef[1] * y;
-            reg[t][m1] = op.coef[2] * x + op.coef[3] * y;
+      if (op.code == TC_GRAD) {
+        if (!lam || !grads) throw Error{QR_ERR_HIP, "model: gradient tap in a single-state stage"};
+        const int gen = (int)op.flags;
+        for (int t = 0; t < T; ++t) {
+          const std::vector<cdv>& p = reg[t];
+          const std::vector<cdv>& l = lreg[t];
+          double acc = 0.0;
+          if (gen == 1 || gen == 2) {
+            const int J = (int)op.a;
+            for (int m0 = 0; m0 < R; ++m0) {
+              if (m0 & (1 << J)) continue;
+              const int m1 = m0 | (1 << J);
+              acc += gen == 1 ? imc(l[m0], p[m1]) + imc(l[m1], p[m0]) : rec(l[m1], p[m0]) - rec(l[m0], p[m1]);
+            }
+          } else {
+            const uint64_t g0 = gidx(t, 0);
+            for (int m = 0; m < R; ++m) {
+              double v = imc(l[m], p[m]);
+              if (gen == 3) {
+                const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : ((g0 >> op.a) & 1);
+                if (bit) v = -v;
+              }
+              acc += v;
+            }
           }
-        } else {
-          for (int m = 0; m < R; ++m) {
-            if (((uint32_t)m & op.ns) != op.ns) continue;
-            const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : ((g0 >> op.a) & 1);
-            if ((op.flags & 1) && !bit) continue;
-            reg[t][m] *= bit ? op.coef[1] : op.coef[0];
+          (*grads)[op.param] += acc;
+        }
+        continue;
+      }
+      for (auto* rs : sets) {
+        std::vector<std::vector<cdv>>& reg = *rs;
+        for (int t = 0; t < T && (op.code == TC_U1 || op.code == TC_D1); ++t) {
+          const uint64_t g0 = gidx(t, 0);
+          const bool lane_ok = (g0 & op.lane_cmask) == op.lane_cmask;
+          if (!lane_ok) continue;
+          if (op.code == TC_U1) {
+            const int J = (int)op.a;
+            for (int m0 = 0; m0 < R; ++m0) {
+              if (m0 & (1 << J)) continue;
+              if (((uint32_t)m0 & op.ns) != op.ns) continue;
+              const int m1 = m0 | (1 << J);
+              const cdv x = reg[t][m0], y = reg[t][m1];
+              reg[t][m0] = op.coef[0] * x + op.coef[1] * y;
+              reg[t][m1] = op.coef[2] * x + op.coef[3] * y;
+            }
+          } else {
+            for (int m = 0; m < R; ++m) {
+              if (((uint32_t)m & op.ns) != op.ns) continue;
+              const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : ((g0 >> op.a) & 1);
+              if ((op.flags & 1) && !bit) continue;
+              reg[t][m] *= bit ? op.coef[1] : op.coef[0];
+            }
           }
         }
       }
@@ -115,23 +158,29 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB) 
             if ((m >> j) & 1) x ^= Q[NT + j];
           return x;
         };
-        std::fill(lds_written.begin(), lds_written.end(), 0);
-        for (int t = 0; t < T; ++t)
-          for (int m = 0; m < R; ++m) {
-            const uint32_t at = lofs(0, t, m);
-            if (at >= lds.size() || lds_written[at]) throw Error{QR_ERR_HIP, "model: LDS write collision"};
-            lds_written[at] = 1;
-            lds[at] = reg[t][m];
-          }
-        for (int t = 0; t < T; ++t)
-          for (int m = 0; m < R; ++m) reg[t][m] = lds[lofs(1, t, m)];
+        for (auto* rs : sets) {
+          std::vector<std::vector<cdv>>& reg = *rs;
+          std::fill(lds_written.begin(), lds_written.end(), 0);
+          for (int t = 0; t < T; ++t)
+            for (int m = 0; m < R; ++m) {
+              const uint32_t at = lofs(0, t, m);
+              if (at >= lds.size() || lds_written[at]) throw Error{QR_ERR_HIP, "model: LDS write collision"};
+              lds_written[at] = 1;
+              lds[at] = reg[t][m];
+            }
+          for (int t = 0; t < T; ++t)
+            for (int m = 0; m < R; ++m) reg[t][m] = lds[lofs(1, t, m)];
+        }
         cur_map = (int)op.a;
       } else if (op.code == TC_SWAPSLOTS) {
         cur_map = (int)op.a;
       }
     }
     for (int t = 0; t < T; ++t)
-      for (int m = 0; m < R; ++m) psi[gidx(t, m)] = reg[t][m];
+      for (int m = 0; m < R; ++m) {
+        psi[gidx(t, m)] = reg[t][m];
+        if (lam) (*lam)[gidx(t, m)] = lreg[t][m];
+      }
   }
 }
 
@@ -157,6 +206,48 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
       else apply_standalone(n, psi, ops[s.standalone]);
     }
     std::memcpy(state, psi.data(), psi.size() * 16);
+    if (sweeps_out) *sweeps_out = stages.size();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return 1;
+  }
+}
+
+// The adjoint sweep on the CPU model: psi = U|0>, lam = O psi given; plans the reverse
+// program with the engine's dual-state planner and replays it. grads: one per parameter.
+int model_adjoint(uint32_t n, double* psi_io, double* lam_io, const qr_program* prog, int tile_bits, int low_bits,
+                  double* grads_out, uint64_t nparams, uint64_t* sweeps_out) {
+  try {
+    auto gates = lower_exact(n, prog);
+    auto ops = lower_fast(n, gates, true);
+    uint64_t np = 0;
+    for (const auto& op : ops)
+      if (op.param >= 0) ++np;
+    if (np != nparams) throw Error{QR_ERR_ARG, "model: parameter count mismatch"};
+    auto rev = adjoint_sweep_ops(ops);
+    PlanParams pp;
+    pp.tile_bits = std::min<int>(tile_bits, (int)n);
+    pp.reg_bits = std::min(4, pp.tile_bits);
+    pp.low_bits = low_bits;
+    auto stages = plan_fused(n, rev, pp, 16, true);
+    std::vector<cdv> psi(1ull << n), lam(1ull << n);
+    std::memcpy(psi.data(), psi_io, psi.size() * 16);
+    std::memcpy(lam.data(), lam_io, lam.size() * 16);
+    std::vector<double> g(np, 0.0);
+    for (const Stage& s : stages) {
+      if (s.tile) {
+        run_tile(n, psi, s, pp.reg_bits, &lam, &g);
+      } else {
+        if (rev[s.standalone].kind == FK_GRAD) throw Error{QR_ERR_HIP, "model: standalone gradient tap"};
+        apply_standalone(n, psi, rev[s.standalone]);
+        apply_standalone(n, lam, rev[s.standalone]);
+      }
+    }
+    for (const auto& op : ops)
+      if (op.param >= 0) grads_out[op.param] = op.gen == 4 ? -g[op.param] : g[op.param];
+    std::memcpy(psi_io, psi.data(), psi.size() * 16);
+    std::memcpy(lam_io, lam.data(), lam.size() * 16);
     if (sweeps_out) *sweeps_out = stages.size();
     return 0;
   } catch (const Error& e) {

@@ -287,3 +287,72 @@ def test_compiled_stages_brickwork_repeat():
         got = st.amplitudes()
         assert np.max(np.abs(got - want)) < TOL_FUSED, rep
     st.close()
+
+
+def _unitary_customs(gates, customs, rng):
+    for i in customs:
+        k = 1 + len(gates[i][2])
+        customs[i] = (W.random_unitary(k, int(rng.integers(1 << 30))).T.copy(), np.ones(1 << k, dtype=np.uint8))
+    return customs
+
+
+def test_adjoint_fused_vs_per_op():
+    """The fused reverse sweep (psi and lam in one tile, gradient taps inside) against the
+    one-launch-per-op reverse sweep, on random programs with every gate kind."""
+    rng = np.random.default_rng(808)
+    for trial in range(10):
+        n = int(rng.integers(2, 15))
+        gates, customs = H.random_gates(n, 60, rng, with_custom=(trial % 2 == 0))
+        customs = _unitary_customs(gates, customs, rng)
+        terms = []
+        for _ in range(4):
+            m = int(rng.integers(1, min(n, 3) + 1))
+            wires = rng.permutation(n)[:m]
+            terms.append((float(rng.uniform(-1, 1)), [(int(w), "XYZ"[int(rng.integers(3))]) for w in wires]))
+        obs = _capi.Observable(terms)
+        p = H.to_program(gates, customs)
+        out = []
+        for fused_backward in (True, False):
+            say("adjoint trial", trial, "n", n, "fused backward", fused_backward)
+            g = _capi.Grad(n)
+            g.set_options(jit=2 if fused_backward else 0)
+            e = g.forward(p, obs)
+            out.append((e, g.backward(1.0)))
+            g.close()
+        assert abs(out[0][0] - out[1][0]) < 1e-12
+        assert np.max(np.abs(out[0][1] - out[1][1])) < 1e-11, trial
+
+
+@pytest.mark.parametrize("precision,tol", [(_capi.QR_C128, 1e-9), (_capi.QR_C64, 1e-5)])
+def test_adjoint_hea_20q(precision, tol):
+    """Config of the adjoint benchmark (20 qubits, 10 layers, 400 parameters) against the
+    parameter-shift rule on the plain complex128 path, for a sample of parameters."""
+    n, L = 20, 10
+    gates, thetas = W.hea(n, L, seed=2020)
+    obs = _capi.Observable(W.hea_observable(n))
+    say("hea 20q adjoint precision", precision)
+    g = _capi.Grad(n, precision)
+    g.forward(W.to_program(gates), obs)
+    grads = g.backward()
+    g.close()
+    assert len(grads) == 400
+    st = _capi.State(n)
+    pidx = [i for i, x in enumerate(gates) if x[0] in ("Rx", "Ry", "Rz")]
+    sample = list(range(0, 400, 37)) + [399]
+    scale = 0.0
+    errs = []
+    for j in sample:
+        i = pidx[j]
+        vals = []
+        for s in (+1, -1):
+            kind, pos, ctl, arg, argi = gates[i]
+            gg = list(gates)
+            gg[i] = (kind, pos, ctl, arg + s * np.pi / 2, argi)
+            st.reset()
+            st.apply(W.to_program(gg))
+            vals.append(st.expectation(obs))
+        ps = 0.5 * (vals[0] - vals[1])
+        scale = max(scale, abs(ps))
+        errs.append(abs(grads[j] - ps))
+    st.close()
+    assert max(errs) / scale < tol, (max(errs), scale)

@@ -33,6 +33,8 @@ def model():
     lib.model_run.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
                               C.POINTER(C.c_uint64)]
     lib.model_error.restype = C.c_char_p
+    lib.model_adjoint.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                  C.c_uint64, C.POINTER(C.c_uint64)]
     return lib
 
 
@@ -93,3 +95,105 @@ def test_model_register_bits(tile_bits, reg_bits):
         got, _ = run_model(n, p, tile_bits, low_bits=3, reg_bits=reg_bits)
         want = H.oracle_simulate(n, p)
         assert np.max(np.abs(got - want)) < 1e-12, (trial, n)
+
+
+# ---- adjoint sweep (dual-state stages with gradient taps) ----------------------------------
+
+def pauli_apply(n, psi, terms):
+    """O psi for O = sum_t c_t P_t by direct index arithmetic (wire w is index bit n-1-w)."""
+    idx = np.arange(1 << n)
+    out = np.zeros_like(psi)
+    for c, fac in terms:
+        v = psi.copy()
+        for w, p in fac:  # rightmost factor acts first; factors on distinct wires commute
+            bit = 1 << (n - 1 - w)
+            b = (idx & bit) != 0
+            if p == "Z":
+                v = np.where(b, -v, v)
+            elif p == "X":
+                v = v[idx ^ bit]
+            else:  # (Y v)_j = -i (-1)^{b_j} v_{j^bit}
+                v = -1j * np.where(b, -1.0, 1.0) * v[idx ^ bit]
+        out += c * v
+    return out
+
+
+def unitary_customs(gates, customs, rng):
+    for i in customs:
+        k = 1 + len(gates[i][2])
+        customs[i] = (W.random_unitary(k, int(rng.integers(1 << 30))).T.copy(), np.ones(1 << k, dtype=np.uint8))
+    return customs
+
+
+def shift_gradients(n, gates, customs, obs):
+    """Parameter-shift rule through the oracle: exact for exp(-i theta G / 2), G^2 = I;
+    the global phase gate has zero gradient."""
+    from quantr_amd import _capi
+    o = _capi.Observable(obs)
+    out = []
+    for i, (kind, pos, ctl, arg, argi) in enumerate(gates):
+        if kind not in ("Rx", "Ry", "Rz", "Phase"):
+            continue
+        if kind == "Phase":
+            out.append(0.0)
+            continue
+        vals = []
+        for sgn in (+1, -1):
+            gg = list(gates)
+            gg[i] = (kind, pos, ctl, arg + sgn * np.pi / 2, argi)
+            psi = H.oracle_simulate(n, H.to_program(gg, customs))
+            vals.append(H.oracle_expectation(n, psi, o))
+        out.append(0.5 * (vals[0] - vals[1]))
+    return np.array(out)
+
+
+def run_model_adjoint(n, gates, customs, obs, tile_bits, low_bits):
+    lib = model()
+    p = H.to_program(gates, customs)
+    psi = H.oracle_simulate(n, p).astype(np.complex128)
+    lam = pauli_apply(n, psi, obs)
+    npar = sum(1 for g in gates if g[0] in ("Rx", "Ry", "Rz", "Phase"))
+    grads = np.zeros(npar)
+    ps = p.c_struct()
+    sw = C.c_uint64()
+    rc = lib.model_adjoint(n, psi.ctypes.data, lam.ctypes.data, C.byref(ps), tile_bits, low_bits,
+                           grads.ctypes.data, npar, C.byref(sw))
+    assert rc == 0, lib.model_error()
+    return grads, psi, sw.value
+
+
+def random_observable(n, rng, terms=5):
+    obs = []
+    for _ in range(terms):
+        m = int(rng.integers(1, min(n, 3) + 1))
+        wires = rng.permutation(n)[:m]
+        obs.append((float(rng.uniform(-1, 1)), [(int(w), "XYZ"[int(rng.integers(3))]) for w in wires]))
+    return obs
+
+
+@pytest.mark.parametrize("tile_bits", [1, 2, 3, 5, 7, 10])
+def test_model_adjoint_random(tile_bits):
+    """Dual-state fused reverse sweep: gradients equal the parameter-shift rule, and psi is
+    returned to |0...0>."""
+    rng = np.random.default_rng(3000 + tile_bits)
+    for trial in range(6):
+        n = int(rng.integers(2, 9))
+        gates, customs = H.random_gates(n, 30, rng, with_custom=(trial % 2 == 0))
+        customs = unitary_customs(gates, customs, rng)
+        obs = random_observable(n, rng)
+        got, psi0, _ = run_model_adjoint(n, gates, customs, obs, tile_bits, int(rng.integers(0, 4)))
+        want = shift_gradients(n, gates, customs, obs)
+        assert np.max(np.abs(got - want)) < 1e-10, (trial, n)
+        e0 = np.zeros(1 << n, dtype=np.complex128)
+        e0[0] = 1.0
+        assert np.max(np.abs(psi0 - e0)) < 1e-10
+
+
+def test_model_adjoint_hea():
+    n, L = 9, 3
+    gates, _ = W.hea(n, L, seed=11)
+    obs = W.hea_observable(n)
+    got, _, sweeps = run_model_adjoint(n, gates, {}, obs, 6, 3)
+    want = shift_gradients(n, gates, {}, obs)
+    assert np.max(np.abs(got - want)) < 1e-10
+    assert sweeps < len(gates) / 3
