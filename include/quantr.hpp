@@ -16,6 +16,7 @@
 #include <complex>
 #include <cstdint>
 #include <cstdio>
+#include <cstring>
 #include <functional>
 #include <map>
 #include <memory>
@@ -426,6 +427,11 @@ struct Program {
     return qr_program{ops.data(), ops.size(), controls.data(), controls.size(),
                       images.data(), images.size(), some.data(), some.size()};
   }
+  bool operator==(const Program& o) const {
+    return ops.size() == o.ops.size() &&
+           (ops.empty() || std::memcmp(ops.data(), o.ops.data(), ops.size() * sizeof(qr_op)) == 0) &&
+           controls == o.controls && images == o.images && some == o.some;
+  }
 };
 
 // Device state owned through a shared pointer (SimulatedCircuit copies share it).
@@ -486,9 +492,8 @@ class SimulatedCircuit {
     return Measurement<Bins>::Observable(std::move(bins));
   }
   // measure_all_without_cache (simulated_circuit.rs:81-114): the first shot from the current
-  // register, every later shot from the circuit re-simulated on |0...0>. Custom gates are
-  // tabulated once per simulation, so every re-simulation yields the same state; it is
-  // simulated once and sampled shots-1 times.
+  // register, every later shot from the circuit re-simulated on |0...0> with its custom
+  // functions called again (2^k calls per custom gate and shot, where the reference makes 2^n).
   Measurement<Bins> measure_all_without_cache(size_t shots);
 
   Measurement<const states::SuperPosition*> get_state() const {
@@ -809,18 +814,31 @@ inline Measurement<SimulatedCircuit::Bins> SimulatedCircuit::measure_all_without
   if (shots == 0) return Measurement<Bins>::Observable(std::move(bins));
   bin_draws({detail::rng().f64()}, bins);
   if (config_progress_) std::printf("Measured state # 1/%zu\n", shots);
-  if (shots > 1) {
-    detail::check(qr_state_reset(dev_->st));  // simulated_circuit.rs:92-97
+  // Every later shot re-simulates the circuit on |0...0>, calling the custom functions again
+  // (simulated_circuit.rs:91-111). Draws taken while the re-tabulated program stays equal to
+  // the one on the device are sampled from that state together.
+  std::vector<double> pending;
+  auto flush = [&]() {
+    if (!pending.empty()) bin_draws(pending, bins);
+    pending.clear();
+  };
+  std::shared_ptr<detail::Program> on_device;
+  for (size_t i = 0; i + 1 < shots; ++i) {
+    auto prog = Circuit::lower_gates(gates_, num_qubits_);
+    if (!on_device || !(*prog == *on_device)) {
+      flush();
+      detail::check(qr_state_reset(dev_->st));
+      qr_program v = prog->view();
+      detail::check(qr_apply_program(dev_->st, &v));
+      host_.reset();
+      on_device = prog;
+      program_ = prog;
+    }
     if (config_progress_) std::printf("Register reset to zero state\n");
-    qr_program v = program_->view();
-    detail::check(qr_apply_program(dev_->st, &v));
-    host_.reset();
-    std::vector<double> u(shots - 1);
-    for (auto& x : u) x = detail::rng().f64();
-    bin_draws(u, bins);
-    if (config_progress_)
-      for (size_t i = 0; i + 1 < shots; ++i) std::printf("Measured state # %zu/%zu\n", i + 2, shots);
+    pending.push_back(detail::rng().f64());
+    if (config_progress_) std::printf("Measured state # %zu/%zu\n", i + 2, shots);
   }
+  flush();
   return Measurement<Bins>::Observable(std::move(bins));
 }
 

@@ -147,7 +147,8 @@ typedef struct qr_options {
   uint32_t reg_bits;     /* amplitudes per thread = 2^reg_bits (0 = auto, 3, 4 or 5)    */
   uint32_t jit;          /* tile sweeps compiled for the circuit at run time (hipRTC):
                             0 never; 1 always; 2 (default) from the second application of
-                            the same circuit structure on this state, or at once for n >= 28 */
+                            the same circuit structure in this process, or at once for
+                            n >= 28. Plans are cached per structure either way.          */
 } qr_options;
 
 typedef struct qr_state qr_state;
