@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--adjoint-layers", type=int, default=10)
     ap.add_argument("--shift-evals", type=int, default=0,
                     help="parameter-shift gradients timed (0 = all parameters)")
+    ap.add_argument("--no-precision", action="store_true", help="skip the complex64 drift report")
+    ap.add_argument("--capacity", action="store_true",
+                    help="also run the 33-qubit complex128 and 34-qubit complex64 capacity circuits")
     return ap.parse_args()
 
 
@@ -155,6 +158,75 @@ def bench_adjoint(args):
             "params": P, "qubits": n, "layers": L}
 
 
+def precision_report(args, prog128_ms):
+    """complex64 against complex128 on the same circuit: max |amplitude difference|, max
+    |probability difference| and |norm - 1| of the complex64 state (norm summed in binary64)."""
+    n = args.qubits
+    prog = W.to_program(W.brickwork(n, depth=args.depth, seed=1234)).c_struct()
+    s128 = _capi.State(n, _capi.QR_C128)
+    s64 = _capi.State(n, _capi.QR_C64)
+    s128.apply(prog)
+    s64.apply(prog)  # compiles the complex64 stages
+    s64.synchronize()
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        s64.reset()
+        s64.apply(prog)
+    s64.synchronize()
+    ms64 = (time.perf_counter() - t0) * 1e3 / reps
+    plan64 = s64.last_plan()
+    norm64 = s64.norm_sqr()
+    da = dp = 0.0
+    chunk = 1 << min(n, 25)
+    for first in range(0, 1 << n, chunk):
+        a = s128.amplitudes_range(first, chunk)
+        b = s64.amplitudes_range(first, chunk)
+        da = max(da, float(np.max(np.abs(a - b))))
+        dp = max(dp, float(np.max(np.abs(np.abs(a) ** 2 - np.abs(b) ** 2))))
+    s128.close()
+    s64.close()
+    log("bench: complex64 %.2f ms/step; max |d amp| %.2e, max |d p| %.2e, |norm-1| %.2e" % (ms64, da, dp, abs(norm64 - 1)))
+    return {"qubits": n, "c64_ms_per_step": round(ms64, 3), "c128_ms_per_step": round(prog128_ms, 3),
+            "c64_hbm_gbs": round((plan64["bytes"] + (1 << n) * 8) / (ms64 * 1e-3) / 1e9, 1),
+            "max_abs_amp_diff": da, "max_abs_prob_diff": dp, "c64_norm_error": abs(norm64 - 1.0)}
+
+
+def capacity(args):
+    """The largest states one GPU holds: 33 qubits complex128 and 34 qubits complex64 (137 GB
+    each), brickwork depth 40 plus a seeded 3-qubit custom gate. Each runs forward (timed on its
+    second application), then the inverse program; the state must return to |0...0>."""
+    out = []
+    for n, prec, name, tol in ((33, _capi.QR_C128, "c128", 1e-9), (34, _capi.QR_C64, "c64", 1e-3)):
+        gates, cust = W.capacity_circuit(n, depth=args.depth)
+        inv, icust = W.inverse(gates, cust)
+        prog = W.to_program(gates, cust).c_struct()
+        iprog = W.to_program(inv, icust).c_struct()
+        log("bench: capacity %d qubits %s (%d gates)" % (n, name, len(gates)))
+        st = _capi.State(n, prec)
+        st.apply(prog)  # first application compiles the stages
+        st.reset()
+        st.synchronize()
+        t0 = time.perf_counter()
+        st.apply(prog)
+        st.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        plan = st.last_plan()
+        norm = st.norm_sqr()
+        st.apply(iprog)
+        a0 = complex(st.amplitudes_range(0, 1)[0])
+        back = st.norm_sqr()
+        st.close()
+        ok = abs(a0 - 1.0) < tol and abs(back - 1.0) < tol
+        log("bench: capacity %dq %s %.1f ms, %d sweeps, norm-1 %.2e, round trip |a0-1| %.2e" % (n, name, ms, plan["sweeps"], norm - 1, abs(a0 - 1)))
+        out.append({"qubits": n, "precision": name, "state_gb": (1 << n) * (16 if prec == _capi.QR_C128 else 8) / 1e9,
+                    "gates": len(gates), "ms": round(ms, 1), "sweeps": plan["sweeps"],
+                    "hbm_gbs": round(plan["bytes"] / (ms * 1e-3) / 1e9, 1),
+                    "gates_per_s": round(len(gates) / (ms * 1e-3), 1), "norm_error": abs(norm - 1.0),
+                    "round_trip_amp0_error": abs(a0 - 1.0), "round_trip_ok": bool(ok)})
+    return out
+
+
 def cpu_baseline(n, depth):
     """The oracle's in-place state vector (C++/OpenMP, all host cores) on a bounded sample:
     the first gates of the same brickwork at the same size, timed per gate."""
@@ -231,6 +303,10 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args)
+    if not args.no_precision:
+        out["precision"] = precision_report(args, ms)
+    if args.capacity:
+        out["capacity"] = capacity(args)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(r["n"], args.depth)

@@ -112,6 +112,55 @@ def random_unitary(k, seed):
     return q * ph  # columns scaled
 
 
+def capacity_circuit(n, depth=40, seed=1234, custom_seed=3):
+    """The brickwork family with one seeded 3-qubit custom gate after the middle layer: a
+    Haar-random 8x8 unitary (QR of a complex Gaussian matrix, Mezzadri 2007) tabulated from its
+    product-state map (input j -> column j, always Some) on 3 seeded distinct wires.
+    Returns (gates, customs) with customs = {index: (images, some)}, images[j][r] = U[r][j]."""
+    import numpy as np
+
+    gates = brickwork(n, depth=depth, seed=seed)
+    per_layer = n + n // 2  # singles + CNots of an even layer; only a cut position is needed
+    cut = min(len(gates), (depth // 2) * per_layer)
+    rng = SplitMix64(custom_seed)
+    ws = []
+    while len(ws) < 3:
+        x = rng.below(n)
+        if x not in ws:
+            ws.append(x)
+    u = random_unitary(3, custom_seed)
+    images = np.ascontiguousarray(u.T)
+    some = np.ones(8, dtype=np.uint8)
+    gates = gates[:cut] + [("Custom", ws[2], (ws[0], ws[1]), 0.0, 0)] + gates[cut:]
+    return gates, {cut: (images, some)}
+
+
+INVERSE_KIND = {"S": "Sdag", "Sdag": "S", "T": "Tdag", "Tdag": "T", "X90": "MX90", "MX90": "X90",
+                "Y90": "MY90", "MY90": "Y90"}
+
+
+def inverse(gates, customs=None):
+    """The inverse program: reversed order, each gate replaced by its inverse (angles negated,
+    S/T and the 90-degree gates by their partners, custom tables by the conjugate transpose).
+    CRk has no inverse among quantr's gates and is rejected."""
+    import numpy as np
+
+    out, out_custom = [], {}
+    for i in range(len(gates) - 1, -1, -1):
+        kind, pos, ctl, arg, argi = gates[i]
+        if kind == "CRk":
+            raise ValueError("CRk has no inverse gate in quantr")
+        if kind == "Custom":
+            images, some = customs[i]
+            out_custom[len(out)] = (np.ascontiguousarray(np.conj(images).T), some)
+            out.append((kind, pos, ctl, 0.0, 0))
+        elif kind in ("Rx", "Ry", "Rz", "Phase", "CR"):
+            out.append((kind, pos, ctl, -arg, argi))
+        else:
+            out.append((INVERSE_KIND.get(kind, kind), pos, ctl, arg, argi))
+    return out, out_custom
+
+
 def uniforms(count, seed):
     rng = SplitMix64(seed)
     return [rng.uniform() for _ in range(count)]
