@@ -230,11 +230,27 @@ qr_status qr_grad_set_options(qr_grad* grad, const qr_options* options);
 
 /* ---- batched serving (no reference counterpart) ---------------------------------------- */
 
-/* Expectation value and full adjoint gradient of many independent programs of uneven
- * qubit counts (complex64 or complex128), in one persistent launch per sweep step.
- * expvals_out[i]; grads_out[grad_offsets[i] ...] receives instance i's gradient;
- * latency_us_out (optional, may be NULL) receives each instance's completion time in
- * microseconds after the launch began. */
+/* Many independent parameterised programs of uneven qubit counts, each with an observable:
+ * qr_batch_run computes every expectation value and full adjoint gradient (as qr_grad_forward
+ * + qr_grad_backward with grad_out = 1) in ONE launch of a HIP graph captured at creation.
+ * Creation plans every instance, compiles the tile stages of every distinct structure and
+ * allocates all states (2 x 2^n amplitudes per instance) up front.
+ * Runs may pass new programs of the same structure (new angles, new custom tables of the same
+ * shape); NULL reuses the programs given at creation. Observables are fixed at creation. */
+typedef struct qr_batch qr_batch;
+qr_status qr_batch_create(uint64_t ninstances, const uint32_t* num_qubits,
+                          const qr_program* programs, const qr_observable* observables,
+                          qr_precision precision, qr_batch** out);
+qr_status qr_batch_destroy(qr_batch* batch);
+/* Parameters (Rx/Ry/Rz/Phase gates) of each instance: counts_out[ninstances]. */
+qr_status qr_batch_param_counts(const qr_batch* batch, uint64_t* counts_out);
+/* expvals_out[i]; grads_out[grad_offsets[i] ...] (grad_offsets NULL: instances packed in
+ * order) receives instance i's gradient; latency_us_out (optional) receives each instance's
+ * completion time in microseconds after the graph began, from the device clock. */
+qr_status qr_batch_run(qr_batch* batch, const qr_program* programs, double* expvals_out,
+                       double* grads_out, const uint64_t* grad_offsets, double* latency_us_out);
+
+/* One-shot form: create, run once, destroy. */
 qr_status qr_batch_grad(uint64_t ninstances, const uint32_t* num_qubits,
                         const qr_program* programs, const qr_observable* observables,
                         qr_precision precision, double* expvals_out, double* grads_out,

@@ -86,6 +86,11 @@ _SIGNATURES = [
     ("qr_grad_num_params", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("qr_grad_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_uint64]),
     ("qr_grad_set_options", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("qr_batch_create", C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
+                                  C.POINTER(C.c_void_p)]),
+    ("qr_batch_destroy", C.c_int, [C.c_void_p]),
+    ("qr_batch_param_counts", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("qr_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("qr_batch_grad", C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
                                 C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     ("qr_profile_enable", C.c_int, [C.c_void_p, C.c_int]),
@@ -374,3 +379,62 @@ def batch_grad(instances, precision=QR_C64):
                               lat.ctypes.data))
     out = [grads[int(offs[i]):int(offs[i]) + counts[i]] for i in range(m)]
     return ev, out, lat
+
+
+class Batch:
+    """Many (num_qubits, program, observable) instances, each with expectation value and full
+    adjoint gradient, executed as one captured graph launch per run (qr_batch)."""
+
+    def __init__(self, instances, precision=QR_C64):
+        self.m = len(instances)
+        self.n = [int(i[0]) for i in instances]
+        nq = np.array(self.n, dtype=np.uint32)
+        obs = (QrObservable * self.m)()
+        self._keep_obs = []
+        for i, (_, _, o) in enumerate(instances):
+            os_ = o.c_struct()
+            self._keep_obs.append(os_)
+            obs[i] = os_
+        progs, keep = self._programs([i[1] for i in instances])
+        self._h = C.c_void_p()
+        check(lib().qr_batch_create(self.m, nq.ctypes.data, C.addressof(progs), C.addressof(obs), precision,
+                                    C.byref(self._h)))
+        self.counts = np.zeros(self.m, dtype=np.uint64)
+        check(lib().qr_batch_param_counts(self._h, self.counts.ctypes.data))
+        self.offsets = np.zeros(self.m, dtype=np.uint64)
+        if self.m:
+            self.offsets[1:] = np.cumsum(self.counts)[:-1]
+
+    def _programs(self, programs):
+        arr = (QrProgram * self.m)()
+        keep = []
+        for i, p in enumerate(programs):
+            ps = p.c_struct() if isinstance(p, Program) else p
+            keep.append(ps)
+            arr[i] = ps
+        return arr, keep
+
+    def run(self, programs=None):
+        """Returns (expvals, list of gradient arrays, latency_us per instance)."""
+        arr = None
+        keep = None
+        if programs is not None:
+            arr, keep = self._programs(programs)
+        grads = np.zeros(int(self.counts.sum()), dtype=np.float64)
+        ev = np.zeros(self.m, dtype=np.float64)
+        lat = np.zeros(self.m, dtype=np.float64)
+        check(lib().qr_batch_run(self._h, C.addressof(arr) if arr is not None else None, ev.ctypes.data,
+                                 grads.ctypes.data if grads.size else None, self.offsets.ctypes.data, lat.ctypes.data))
+        out = [grads[int(self.offsets[i]):int(self.offsets[i] + self.counts[i])] for i in range(self.m)]
+        return ev, out, lat
+
+    def close(self):
+        if self._h:
+            lib().qr_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

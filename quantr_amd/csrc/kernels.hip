@@ -712,6 +712,63 @@ __global__ __launch_bounds__(256) void k_gen_partial(const typename CT<R>::T* __
   if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
 }
 
+// lam[o] (+)= sum_t c_t (-1)^{popcount(j & zm_t)} psi[j], j = o ^ xm: every Pauli term of one
+// X-mask group in one pass (terms read with uniform loads).
+template <typename R>
+__global__ __launch_bounds__(256) void k_pauli_group(typename CT<R>::T* __restrict__ lam,
+                                                     const typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                     uint64_t xm, const PauliZTerm* __restrict__ terms, int nterms,
+                                                     int accumulate) {
+  using C = typename CT<R>::T;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < dim; o += stride) {
+    const uint64_t j = o ^ xm;
+    double fr = 0.0, fi = 0.0;
+    for (int t = 0; t < nterms; ++t) {
+      const bool neg = __popcll(j & terms[t].zm) & 1;
+      fr += neg ? -terms[t].cr : terms[t].cr;
+      fi += neg ? -terms[t].ci : terms[t].ci;
+    }
+    double yr = 0.0, yi = 0.0;
+    if (nterms) {
+      const double pr = (double)psi[j].x, pi = (double)psi[j].y;
+      yr = fr * pr - fi * pi;
+      yi = fr * pi + fi * pr;
+    }
+    if (accumulate) {
+      yr += (double)lam[o].x;
+      yi += (double)lam[o].y;
+    }
+    C w;
+    w.x = (R)yr;
+    w.y = (R)yi;
+    lam[o] = w;
+  }
+}
+
+// Partial sums of Re sum_j conj(a_j) b_j, one per workgroup.
+template <typename R>
+__global__ __launch_bounds__(256) void k_dot_re_partial(const typename CT<R>::T* __restrict__ a,
+                                                        const typename CT<R>::T* __restrict__ b, uint64_t dim,
+                                                        double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride)
+    acc += (double)a[j].x * (double)b[j].x + (double)a[j].y * (double)b[j].y;
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_stamp(uint64_t* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = wall_clock64();
+}
+
 // Row sums of a partial-sum table in a fixed order (one workgroup per row): each thread adds
 // a strided subset, then a tree over the workgroup.
 __global__ __launch_bounds__(256) void k_sum_rows(const double* __restrict__ part, uint64_t ncols,
@@ -994,6 +1051,30 @@ hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64
     dev::k_gen_partial<double><<<nblocks, 256, 0, s>>>((const double2*)lam, (const double2*)psi, dim, t, gen, part);
   else
     dev::k_gen_partial<float><<<nblocks, 256, 0, s>>>((const float2*)lam, (const float2*)psi, dim, t, gen, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_pauli_group(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, const PauliZTerm* terms,
+                              int nterms, int accumulate, hipStream_t s) {
+  const unsigned g = grid_for(dim, 256, 1u << 18);
+  if (prec == 0)
+    dev::k_pauli_group<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, xm, terms, nterms, accumulate);
+  else
+    dev::k_pauli_group<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, xm, terms, nterms, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
+                                 hipStream_t s) {
+  if (prec == 0)
+    dev::k_dot_re_partial<double><<<nblocks, 256, 0, s>>>((const double2*)a, (const double2*)b, dim, part);
+  else
+    dev::k_dot_re_partial<float><<<nblocks, 256, 0, s>>>((const float2*)a, (const float2*)b, dim, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_stamp(uint64_t* out, hipStream_t s) {
+  dev::k_stamp<<<1, 64, 0, s>>>(out);
   return hipGetLastError();
 }
 

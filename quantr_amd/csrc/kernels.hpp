@@ -108,6 +108,20 @@ hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim
                               double cr, double ci, int accumulate, hipStream_t s);
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s);
+// One Pauli-sum term of an X-mask group: coefficient (with i^ny folded in) times
+// (-1)^{popcount(j & zm)}.
+struct PauliZTerm {
+  uint64_t zm;
+  double cr, ci;
+};
+// lam[o] (+)= sum_t c_t (-1)^{popcount(j & zm_t)} psi[j], j = o ^ xm; nterms = 0 writes zeros.
+hipError_t launch_pauli_group(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, const PauliZTerm* terms,
+                              int nterms, int accumulate, hipStream_t s);
+// part[b] = partial sums of Re sum_j conj(a_j) b_j over nblocks blocks.
+hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
+                                 hipStream_t s);
+// out[0] = the device wall clock (constant-rate counter).
+hipError_t launch_stamp(uint64_t* out, hipStream_t s);
 // out[r] = sum of part[r * ncols + 0 .. ncols-1] (fixed order), r < nrows.
 hipError_t launch_sum_rows(const double* part, uint64_t ncols, uint32_t nrows, double* out, hipStream_t s);
 hipError_t launch_f64_to_c64(float2* dst, const double2* src, uint64_t n, hipStream_t s);

@@ -356,3 +356,82 @@ def test_adjoint_hea_20q(precision, tol):
         errs.append(abs(grads[j] - ps))
     st.close()
     assert max(errs) / scale < tol, (max(errs), scale)
+
+
+def _random_instance(rng, n, with_custom):
+    gates, customs = H.random_gates(n, 40, rng, with_custom=with_custom)
+    customs = _unitary_customs(gates, customs, rng)
+    terms = []
+    for _ in range(3):
+        m = int(rng.integers(1, min(n, 3) + 1))
+        wires = rng.permutation(n)[:m]
+        terms.append((float(rng.uniform(-1, 1)), [(int(w), "XYZ"[int(rng.integers(3))]) for w in wires]))
+    return gates, customs, terms
+
+
+def test_batch_matches_single_instances():
+    """One captured graph over uneven instances against one gradient context per instance."""
+    rng = np.random.default_rng(909)
+    inst, ref = [], []
+    for i in range(12):
+        n = int(rng.integers(1, 15))
+        gates, customs, terms = _random_instance(rng, n, with_custom=(i % 3 == 0))
+        p = H.to_program(gates, customs)
+        obs = _capi.Observable(terms)
+        inst.append((n, p, obs))
+        g = _capi.Grad(n)
+        e = g.forward(p, obs)
+        ref.append((e, g.backward(1.0)))
+        g.close()
+    say("batch of", len(inst))
+    b = _capi.Batch(inst, _capi.QR_C128)
+    ev, grads, lat = b.run()
+    for i, (e, gr) in enumerate(ref):
+        assert abs(ev[i] - e) < 1e-11, i
+        assert len(grads[i]) == len(gr)
+        assert np.max(np.abs(grads[i] - gr), initial=0.0) < 1e-11, i
+    assert np.all(lat > 0)
+    b.close()
+
+
+def test_batch_rerun_new_angles_and_c64():
+    """A second run of the captured graph with new angles, and the complex64 batch against the
+    complex128 one."""
+    rng = np.random.default_rng(910)
+    sizes = [3, 7, 10, 12, 13, 16]
+    insts = []
+    for n in sizes:
+        gates, thetas = W.hea(n, 3, seed=int(rng.integers(1 << 30)))
+        insts.append((n, gates))
+    obs = [_capi.Observable(W.hea_observable(n)) for n in sizes]
+    b128 = _capi.Batch([(n, W.to_program(g), o) for (n, g), o in zip(insts, obs)], _capi.QR_C128)
+    b64 = _capi.Batch([(n, W.to_program(g), o) for (n, g), o in zip(insts, obs)], _capi.QR_C64)
+    for rep in range(2):
+        new = []
+        for n, gates in insts:
+            gg = [(k, p, c, (a + 0.37 * (rep + 1)) if k in ("Ry", "Rz") else a, ai) for (k, p, c, a, ai) in gates]
+            new.append((n, gg))
+        progs = [W.to_program(g) for _, g in new]
+        say("batch rerun", rep)
+        ev, grads, _ = b128.run(progs)
+        ev64, grads64, _ = b64.run(progs)
+        for i, (n, g) in enumerate(new):
+            gc = _capi.Grad(n)
+            e = gc.forward(W.to_program(g), obs[i])
+            gr = gc.backward()
+            gc.close()
+            assert abs(ev[i] - e) < 1e-11
+            assert np.max(np.abs(grads[i] - gr)) < 1e-11
+            scale = max(1.0, float(np.max(np.abs(gr))))
+            assert np.max(np.abs(grads64[i] - gr)) / scale < 1e-4, (i, n)
+    b128.close()
+    b64.close()
+
+
+def test_batch_rejects_structure_change():
+    n = 4
+    gates, _ = W.hea(n, 2, seed=1)
+    b = _capi.Batch([(n, W.to_program(gates), _capi.Observable(W.hea_observable(n)))], _capi.QR_C128)
+    with pytest.raises(_capi.QrError):
+        b.run([W.to_program(gates[:-1])])
+    b.close()
