@@ -51,6 +51,8 @@ def parse():
     ap.add_argument("--shift-evals", type=int, default=0,
                     help="parameter-shift gradients timed (0 = all parameters)")
     ap.add_argument("--no-precision", action="store_true", help="skip the complex64 drift report")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
+    ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--capacity", action="store_true",
                     help="also run the 33-qubit complex128 and 34-qubit complex64 capacity circuits")
     return ap.parse_args()
@@ -227,6 +229,101 @@ def capacity(args):
     return out
 
 
+def batch_instances(count, seed=512):
+    """count HEA instances: n ~ U{10..24}, L ~ U{2..12}, theta ~ U[-pi, pi), each its own seed."""
+    rng = W.SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = 10 + rng.below(15)
+        L = 2 + rng.below(11)
+        gates, thetas = W.hea(n, L, seed=rng.next_u64())
+        out.append((n, L, gates, thetas))
+    return out
+
+
+def bench_batch(args):
+    """Batched serving: every instance's expectation value and full adjoint gradient from one
+    graph launch (complex64), against the plain GPU parameter-shift path on one parameter per
+    instance (complex64 timing, complex128 reference for the error)."""
+    inst = batch_instances(args.batch_instances)
+    P = sum(len(t) for _, _, _, t in inst)
+    obs = {n: _capi.Observable(W.hea_observable(n)) for n in range(10, 25)}
+    items = [(n, W.to_program(g), obs[n]) for n, _, g, _ in inst]
+    log("bench: batch of %d instances, %d parameters" % (len(inst), P))
+    t0 = time.perf_counter()
+    # compile the stages qubit count by qubit count (one small batch per count), so that the
+    # one-time setup reports progress; the full batch then finds every stage compiled
+    for n in sorted({x[0] for x in inst}):
+        reps = {}
+        for m, L, g, _ in inst:
+            if m == n and L not in reps:
+                reps[L] = (m, W.to_program(g), obs[m])
+        warm = _capi.Batch(list(reps.values()), _capi.QR_C64)
+        warm.close()
+        log("bench: batch stages compiled for %d qubits (%d layer counts), %.1f s" % (n, len(reps), time.perf_counter() - t0))
+    b = _capi.Batch(items, _capi.QR_C64)
+    setup = time.perf_counter() - t0
+    b.run()  # warm-up
+    reps = 3
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ev, grads, lat = b.run()
+    t_run = (time.perf_counter() - t0) / reps
+    b.close()
+    # plain parameter shift, one parameter per instance
+    rng = W.SplitMix64(77)
+    st64 = {}
+    st128 = {}
+    t_ps = 0.0
+    err_num = err_den = 0.0
+    for i, (n, L, gates, thetas) in enumerate(inst):
+        pidx = [k for k, g in enumerate(gates) if g[0] in ("Ry", "Rz")]
+        j = rng.below(len(pidx))
+        k = pidx[j]
+        progs = []
+        for sgn in (+1, -1):
+            gg = list(gates)
+            kind, pos, ctl, a, ai = gg[k]
+            gg[k] = (kind, pos, ctl, a + sgn * math.pi / 2, ai)
+            progs.append(W.to_program(gg).c_struct())
+        if n not in st64:
+            st64[n] = _capi.State(n, _capi.QR_C64)
+            st128[n] = _capi.State(n, _capi.QR_C128)
+        for p in progs:  # untimed: plans and compiled stages of this structure
+            st64[n].reset()
+            st64[n].apply(p)
+        vals = []
+        ts = time.perf_counter()
+        for p in progs:
+            st64[n].reset()
+            st64[n].apply(p)
+            vals.append(st64[n].expectation(obs[n]))
+        t_ps += (time.perf_counter() - ts) * len(thetas)  # this instance's gradient, all parameters
+        ref = []
+        for p in progs:
+            st128[n].reset()
+            st128[n].apply(p)
+            ref.append(st128[n].expectation(obs[n]))
+        g_ref = 0.5 * (ref[0] - ref[1])
+        err_num = max(err_num, abs(grads[i][j] - g_ref))
+        err_den = max(err_den, abs(g_ref))
+    for d in (st64, st128):
+        for s_ in d.values():
+            s_.close()
+    lat_sorted = np.sort(lat)
+    out = {"instances": len(inst), "params": P, "precision": "c64", "setup_s": round(setup, 2),
+           "run_ms": round(t_run * 1e3, 3), "grads_per_s": round(P / t_run, 1),
+           "latency_us_p50": round(float(np.percentile(lat_sorted, 50)), 1),
+           "latency_us_p99": round(float(np.percentile(lat_sorted, 99)), 1),
+           "latency_us_max": round(float(lat_sorted[-1]), 1),
+           "param_shift_grads_per_s": round(P / t_ps, 1), "batch_vs_param_shift": round(t_ps / t_run, 1),
+           "rel_grad_error_vs_c128_shift": err_num / max(err_den, 1e-300)}
+    log("bench: batch %.1f ms per run (%.0f grads/s), p50 %.0f us, p99 %.0f us; parameter shift %.0f grads/s; "
+        "ratio %.0f; rel err %.2e" % (t_run * 1e3, P / t_run, out["latency_us_p50"], out["latency_us_p99"],
+                                      P / t_ps, t_ps / t_run, out["rel_grad_error_vs_c128_shift"]))
+    return out
+
+
 def cpu_baseline(n, depth):
     """The oracle's in-place state vector (C++/OpenMP, all host cores) on a bounded sample:
     the first gates of the same brickwork at the same size, timed per gate."""
@@ -303,6 +400,8 @@ def main():
     }
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args)
+    if not args.no_batch:
+        out["batch"] = bench_batch(args)
     if not args.no_precision:
         out["precision"] = precision_report(args, ms)
     if args.capacity:

@@ -93,6 +93,7 @@ PlanParams fwd_params(uint32_t n, int prec) {
   pp.tile_bits = std::min<int>(prec == 0 ? 12 : 13, (int)n);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
   pp.low_bits = prec == 0 ? 3 : 4;
+  if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
   return pp;
 }
 
@@ -101,6 +102,7 @@ PlanParams dual_params(uint32_t n, int prec) {
   pp.tile_bits = std::min<int>(prec == 0 ? 10 : 11, (int)n);
   pp.reg_bits = std::min(4, pp.tile_bits);
   pp.low_bits = prec == 0 ? 3 : 4;
+  if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
   return pp;
 }
 

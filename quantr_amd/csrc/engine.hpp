@@ -116,7 +116,13 @@ struct PlanParams {
   int tile_bits = 12;
   int reg_bits = 4;
   int low_bits = 4;
+  int max_ops = 0;  // ops per tile stage at most (0 = no limit)
 };
+
+// Stage op limit when one tile covers the whole state: such a stage could otherwise absorb the
+// entire program into one compiled kernel (long compiles, large code); a split costs one launch
+// on a cache-resident state.
+constexpr int kWholeStateStageOps = 96;
 
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
                               int bytes_per_amp, bool dual);

@@ -185,7 +185,9 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
         } else {
           o << "    acc = qr::dev::tap_diag<R, " << RB << ", -1>(l, r);\n";
         }
-        o << "    qr::dev::tap_store<" << TPB << ">(acc, part + (uint64_t)" << op.param << "u * nslots, tile, tid); }\n";
+        // the parameter's row comes from the op's coefficient slot, so that stages differing
+        // only in parameter numbering share one kernel
+        o << "    qr::dev::tap_store<" << TPB << ">(acc, part + (uint64_t)cf[" << 4 * i << "].x * nslots, tile, tid); }\n";
         break;
       }
       case TC_REMAP: {

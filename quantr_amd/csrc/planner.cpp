@@ -221,8 +221,7 @@ void plan_key_append(const FastOp& op, std::vector<uint64_t>& key) {
   }
   uint64_t bits = 0;
   for (int q = 0; q < op.k && q < 8; ++q) bits |= (uint64_t)(op.bits[q] & 0xFF) << (8 * q);
-  key.push_back((uint64_t)op.kind | ((uint64_t)op.k << 8) | ((uint64_t)op.gen << 16) | (cls << 24) |
-                ((uint64_t)(op.kind == FK_GRAD ? op.param : 0) << 32));
+  key.push_back((uint64_t)op.kind | ((uint64_t)op.k << 8) | ((uint64_t)op.gen << 16) | (cls << 24));
   key.push_back(op.cmask);
   key.push_back(bits);
 }
@@ -274,6 +273,12 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
 
     std::vector<int> taken, left;
     absorb(remaining, mk, eligible, B, &taken, &left);
+    if (pp.max_ops > 0 && (int)taken.size() > pp.max_ops) {
+      // defer the tail; the remaining list stays in program order
+      left.insert(left.end(), taken.begin() + pp.max_ops, taken.end());
+      taken.resize(pp.max_ops);
+      std::sort(left.begin(), left.end());
+    }
 
     Stage s;
     s.tile = true;

@@ -34,6 +34,7 @@ PlanParams plan_params(const qr_state* st) {
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
   if ((st->opt.reg_bits == 5 || st->opt.reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = (int)st->opt.reg_bits;
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
+  if (pp.tile_bits == (int)st->n) pp.max_ops = kWholeStateStageOps;
   return pp;
 }
 
@@ -71,6 +72,9 @@ std::vector<uint8_t> pack_blob(const std::vector<FastOp>& ops, const std::vector
           const std::vector<cd>& m = ops[h.src].m;
           for (int q = 0; q < 4; ++q)
             c[4 * i + q] = q < (int)m.size() ? make_double2(m[q].real(), m[q].imag()) : make_double2(0.0, 0.0);
+        } else if (h.code == TC_GRAD) {  // parameter row of the tap (an exact small integer)
+          c[4 * i] = make_double2((double)h.param, 0.0);
+          for (int q = 1; q < 4; ++q) c[4 * i + q] = make_double2(0.0, 0.0);
         } else {
           for (int q = 0; q < 4; ++q) c[4 * i + q] = make_double2(h.coef[q].real(), h.coef[q].imag());
         }
@@ -197,7 +201,8 @@ std::shared_ptr<CachedPlan> get_plan(const qr_state* st, const std::vector<FastO
 std::shared_ptr<CachedPlan> get_plan_for(uint32_t n, int prec, const std::vector<FastOp>& ops, const PlanParams& pp,
                                          bool fused, bool dual, bool persistent) {
   std::vector<uint64_t> key{n, (uint64_t)prec, (uint64_t)pp.tile_bits, (uint64_t)pp.reg_bits,
-                            (uint64_t)pp.low_bits, (uint64_t)fused, (uint64_t)dual, (uint64_t)persistent};
+                            (uint64_t)pp.low_bits, (uint64_t)pp.max_ops, (uint64_t)fused, (uint64_t)dual,
+                            (uint64_t)persistent};
   key.reserve(8 + 3 * ops.size());
   for (const FastOp& op : ops) plan_key_append(op, key);
   {
@@ -572,6 +577,7 @@ PlanParams dual_plan_params(const qr_state* st) {
   pp.tile_bits = std::min<int>(k, (int)st->n);
   pp.reg_bits = std::min(4, pp.tile_bits);
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
+  if (pp.tile_bits == (int)st->n) pp.max_ops = kWholeStateStageOps;
   return pp;
 }
 

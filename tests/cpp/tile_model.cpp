@@ -198,6 +198,7 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
     pp.reg_bits = pp.tile_bits < 4 ? pp.tile_bits : 4;
     if ((reg_bits == 5 || reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = reg_bits;
     pp.low_bits = low_bits;
+    if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);
@@ -230,6 +231,7 @@ int model_adjoint(uint32_t n, double* psi_io, double* lam_io, const qr_program* 
     pp.tile_bits = std::min<int>(tile_bits, (int)n);
     pp.reg_bits = std::min(4, pp.tile_bits);
     pp.low_bits = low_bits;
+    if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
     auto stages = plan_fused(n, rev, pp, 16, true);
     std::vector<cdv> psi(1ull << n), lam(1ull << n);
     std::memcpy(psi.data(), psi_io, psi.size() * 16);

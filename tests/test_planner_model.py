@@ -74,6 +74,17 @@ def test_model_brickwork_and_qft():
     assert np.max(np.abs(got - H.oracle_simulate(10, p))) < 1e-12
 
 
+def test_model_whole_state_stage_limit():
+    """A tile covering the whole state stops at the per-stage op limit; the split keeps order."""
+    rng = np.random.default_rng(45)
+    n = 6
+    gates, customs = H.random_gates(n, 400, rng, with_custom=True)
+    p = H.to_program(gates, customs)
+    got, sweeps = run_model(n, p, 12)
+    assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12
+    assert sweeps >= 4
+
+
 def test_model_swap_heavy():
     rng = np.random.default_rng(44)
     for trial in range(10):
@@ -187,6 +198,17 @@ def test_model_adjoint_random(tile_bits):
         e0 = np.zeros(1 << n, dtype=np.complex128)
         e0[0] = 1.0
         assert np.max(np.abs(psi0 - e0)) < 1e-10
+
+
+def test_model_adjoint_long_whole_state():
+    rng = np.random.default_rng(46)
+    n = 5
+    gates, customs = H.random_gates(n, 200, rng, with_custom=True)
+    customs = unitary_customs(gates, customs, rng)
+    obs = random_observable(n, rng)
+    got, _, sweeps = run_model_adjoint(n, gates, customs, obs, 10, 2)
+    assert np.max(np.abs(got - shift_gradients(n, gates, customs, obs))) < 1e-10
+    assert sweeps >= 3
 
 
 def test_model_adjoint_hea():
