@@ -54,6 +54,24 @@ enum U1Form : int {
   U1_REAL = 2,   // real entries (H, Ry)
 };
 
+// Kernel forms of a register-slot 2x2 (TileOpHost::b of TC_U1): the U1Form values, then
+enum TileForm : int {
+  TF_X = 3,       // [[0,1],[1,0]]: a relabel
+  TF_RX = 4,      // real diagonal, imaginary off-diagonal (Rx)
+  TF_HAD = 5,     // [[1,1],[1,-1]] (H with its scalar factored out)
+  TF_NEGSWAP = 6, // [[0,-1],[1,0]] (Y, Y90, MY90 with their scalar factored out)
+};
+
+// Exact value classes of an op, used to factor scalars out of uncontrolled gates.
+enum OpSub : int {
+  SUB_NONE = 0,
+  SUB_HAD = 1,     // DENSE1: m00 = m01 = m10 = -m11, real
+  SUB_XLIKE = 2,   // DENSE1: anti-diagonal, m01 = m10 (X90 = -iX, MX90 = iX, X)
+  SUB_YLIKE = 3,   // DENSE1: anti-diagonal, m01 = -m10 (Y = i[[0,-1],[1,0]], Y90, MY90)
+  SUB_RX = 4,      // DENSE1: real diagonal, imaginary off-diagonal
+  SUB_D0 = 5,      // DIAG1: m0 != 1 (factor m0 out, leaving diag(1, m1/m0))
+};
+
 struct FastOp {
   int kind = FK_DENSE1;
   int k = 1;
@@ -61,6 +79,7 @@ struct FastOp {
   uint64_t cmask = 0;   // control bits that must be 1
   std::vector<cd> m;    // DENSE1: 4 row-major; DIAG1: 2; DENSEK: dim*dim; DIAGK: dim
   int form = U1_GENERAL;
+  int sub = SUB_NONE;
   // Differentiation: index of the angle parameter (-1 if constant) and its generator
   // (1 = X, 2 = Y, 3 = Z, 4 = identity / global phase): U = exp(-i theta G / 2), or
   // exp(+i theta / 2) I for the global phase.
@@ -79,6 +98,8 @@ std::vector<FastOp> lower_fast(uint32_t n, const std::vector<ExactGate>& gates, 
 // Human-readable form of an op (Rust Debug style, e.g. "CNot(2)").
 std::string gate_debug(const qr_op& op, const uint32_t* controls);
 
+// Sets form and sub of a DENSE1 / DIAG1 op from its matrix.
+void classify(FastOp& op);
 // Adjoint of a fast op (for the reverse sweep).
 FastOp adjoint(const FastOp& op);
 // The reverse sweep of a parameterised program as one op list over (psi, lam): from the
@@ -97,8 +118,14 @@ struct TileOpHost {
   uint32_t swz = 0;      // REMAP: index of the LDS swizzle table
   uint32_t param = 0;    // GRAD: parameter row of the partial-sum table
   int src = -1;          // U1 / D1: index of the op whose matrix supplies the coefficients
+  int fold = 0;          // scalar folding (FoldMode)
   cd coef[4];
 };
+
+// How an op takes part in a stage's scalar folding: its factor is divided out of its
+// coefficients (FOLD_M00: m00, FOLD_M10: m10, FOLD_D0: d0), and the product of the stage's
+// factors is multiplied into one general-form op (FOLD_ABSORB) or applied by TC_SCALE.
+enum FoldMode : int { FOLD_NONE = 0, FOLD_M00 = 1, FOLD_M10 = 2, FOLD_D0 = 3, FOLD_ABSORB = 4, FOLD_SCALE = 5 };
 
 struct Stage {
   bool tile = true;            // fused tile sweep, or a standalone op
@@ -130,5 +157,8 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
 // value-derived forms), so that programs with equal keys share one plan; coefficients are
 // then taken from each program's own ops (TileOpHost::src).
 void plan_key_append(const FastOp& op, std::vector<uint64_t>& key);
+// The 4 coefficients of every op of a tile stage (c[4i..4i+3]), taken from the program's own
+// ops with the stage's scalar folding applied.
+void stage_coefficients(const Stage& s, const std::vector<FastOp>& ops, std::vector<cd>& c);
 
 }  // namespace qr

@@ -127,6 +127,11 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             o << " qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(" << v
               << ", u, " << ns << ");";
           o << " }\n";
+        } else if (op.b == TF_X) {  // controlled X: per-lane select, no arithmetic
+          o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";";
+          for (const auto& v : sets)
+            o << " qr::dev::u1_xsel<R, " << RB << ", " << op.a << ", " << has_ns << ">(" << v << ", ok, " << ns << ");";
+          o << " }\n";
         } else {
           o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"
             << "    qr::dev::Coef4 u = " << coef(i) << ";\n"
@@ -232,6 +237,12 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
       case TC_SWAPSLOTS:
         mi = (int)op.a;
         o << "  g0 = " << thread_part(mi) << ";\n";
+        break;
+      case TC_SCALE:
+        o << "  { const double2 f = cf[" << 4 * i << "];";
+        for (const auto& v : sets)
+          o << " qr::dev::scale_all<R, " << RB << ", 0, false>(" << v << ", (R)f.x, (R)f.y, 0u);";
+        o << " }\n";
         break;
       default:
         break;

@@ -423,6 +423,9 @@ __global__ __launch_bounds__(1 << (K - RB)) void k_tile(typename CT<R>::T* __res
         case TC_SWAPSLOTS:
           ctx.setup(a.maps + (size_t)K * op.map, base, tid);
           break;
+        case TC_SCALE:
+          scale_all<R, RB, 0, false>(cur, (R)cf.c[0].x, (R)cf.c[0].y, 0);
+          break;
         default:
           break;
       }

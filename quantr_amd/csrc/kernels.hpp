@@ -42,6 +42,7 @@ enum TileCode : uint32_t {
   TC_D1 = 3,         // a = global bit, b = register slot or 0xFF, coef -> 2 complex, c = only |1>
   TC_SWAPSLOTS = 4,  // a = index of the new slot map (relabel, no data movement)
   TC_GRAD = 5,       // compiled dual-state stages only: gradient partial of one parameter
+  TC_SCALE = 6,      // every amplitude times coef 0 (the stage's folded scalars)
 };
 
 // One interpreted op of a tile program (32 bytes, read with scalar loads).

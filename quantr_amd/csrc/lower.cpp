@@ -351,12 +351,8 @@ std::vector<FastOp> lower_fast(uint32_t n, const std::vector<ExactGate>& gates, 
       f.kind = K == 1 ? FK_DENSE1 : FK_DENSEK;
       f.m = A;
       f.f_touch = 1.0 / (double)(1ull << nc);
-      if (K == 1) {
-        if (is_zero(A[0]) && is_zero(A[3])) f.form = U1_PERM;
-        else if (A[0].imag() == 0 && A[1].imag() == 0 && A[2].imag() == 0 && A[3].imag() == 0)
-          f.form = U1_REAL;
-      }
     }
+    classify(f);
     if (want_params) {
       switch (e.kind) {
         case QR_GATE_RX: f.param = pcount++; f.gen = 1; break;
@@ -395,6 +391,27 @@ std::vector<FastOp> adjoint_sweep_ops(const std::vector<FastOp>& ops) {
   return rev;
 }
 
+void classify(FastOp& f) {
+  f.form = U1_GENERAL;
+  f.sub = SUB_NONE;
+  if (f.kind == FK_DIAG1) {
+    if (!is_one(f.m[0])) f.sub = SUB_D0;
+    return;
+  }
+  if (f.kind != FK_DENSE1) return;
+  const std::vector<cd>& A = f.m;
+  if (is_zero(A[0]) && is_zero(A[3])) {
+    f.form = U1_PERM;
+    if (A[1] == A[2]) f.sub = SUB_XLIKE;
+    else if (A[1] == -A[2]) f.sub = SUB_YLIKE;
+  } else if (A[0].imag() == 0 && A[1].imag() == 0 && A[2].imag() == 0 && A[3].imag() == 0) {
+    f.form = U1_REAL;
+    if (A[0] == A[1] && A[0] == A[2] && A[3] == -A[0]) f.sub = SUB_HAD;
+  } else if (A[0].imag() == 0 && A[3].imag() == 0 && A[1].real() == 0 && A[2].real() == 0) {
+    f.sub = SUB_RX;
+  }
+}
+
 FastOp adjoint(const FastOp& op) {
   FastOp a = op;
   if (op.kind == FK_DIAG1 || op.kind == FK_DIAGK) {
@@ -403,13 +420,8 @@ FastOp adjoint(const FastOp& op) {
     const int dim = 1 << op.k;
     for (int r = 0; r < dim; ++r)
       for (int j = 0; j < dim; ++j) a.m[r * dim + j] = std::conj(op.m[j * dim + r]);
-    if (op.k == 1) {
-      if (is_zero(a.m[0]) && is_zero(a.m[3])) a.form = U1_PERM;
-      else if (a.m[0].imag() == 0 && a.m[1].imag() == 0 && a.m[2].imag() == 0 && a.m[3].imag() == 0)
-        a.form = U1_REAL;
-      else a.form = U1_GENERAL;
-    }
   }
+  classify(a);
   return a;
 }
 

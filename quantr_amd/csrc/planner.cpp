@@ -221,13 +221,14 @@ void plan_key_append(const FastOp& op, std::vector<uint64_t>& key) {
   }
   uint64_t bits = 0;
   for (int q = 0; q < op.k && q < 8; ++q) bits |= (uint64_t)(op.bits[q] & 0xFF) << (8 * q);
-  key.push_back((uint64_t)op.kind | ((uint64_t)op.k << 8) | ((uint64_t)op.gen << 16) | (cls << 24));
+  key.push_back((uint64_t)op.kind | ((uint64_t)op.k << 8) | ((uint64_t)op.gen << 16) | (cls << 24) |
+                ((uint64_t)op.sub << 40));
   key.push_back(op.cmask);
   key.push_back(bits);
 }
 
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
-                              int bytes_per_amp, bool /*dual*/) {
+                              int bytes_per_amp, bool dual) {
   std::vector<Stage> stages;
   const int N = (int)ops.size();
   std::vector<Masks> mk(N);
@@ -394,7 +395,22 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           t.a = (uint32_t)(slot_of_pos(p) - nts);
           for (int q = 0; q < 4; ++q) t.coef[q] = op.m[q];
           int form = op.form;
-          if (form == U1_PERM && op.m[1] == cd(1, 0) && op.m[2] == cd(1, 0)) form = 3;  // X
+          const bool x = form == U1_PERM && op.m[1] == cd(1, 0) && op.m[2] == cd(1, 0);
+          const bool free = !dual && t.ns == 0 && t.lane_cmask == 0;  // uncontrolled: scalar is global
+          if (x) {
+            form = TF_X;
+          } else if (op.sub == SUB_RX) {
+            form = TF_RX;
+          } else if (free && op.sub == SUB_HAD) {
+            form = TF_HAD;
+            t.fold = FOLD_M00;
+          } else if (free && op.sub == SUB_XLIKE) {
+            form = TF_X;
+            t.fold = FOLD_M10;
+          } else if (free && op.sub == SUB_YLIKE) {
+            form = TF_NEGSWAP;
+            t.fold = FOLD_M10;
+          }
           t.b = (uint32_t)form;
           t.src = i;
           s.ops.push_back(t);
@@ -412,6 +428,9 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
             if (op.m[1] == cd(-1, 0)) t.flags |= 1u << 4;
             else if (op.m[1] == cd(0, 1)) t.flags |= 2u << 4;
             else if (op.m[1] == cd(0, -1)) t.flags |= 3u << 4;
+          } else if (!dual && t.ns == 0 && t.lane_cmask == 0 && op.sub == SUB_D0) {
+            t.flags = 1;  // d0 factored out: diag(1, d1/d0)
+            t.fold = FOLD_D0;
           }
           s.ops.push_back(t);
         } else if (op.kind == FK_GRAD) {
@@ -446,6 +465,25 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         throw Error{QR_ERR_UNSUPPORTED, "internal planner error: no progress inside a tile"};
       emit_remap(make_map(R2));
     }
+    // ---- folded scalars: one general-form uncontrolled op absorbs their product, else a
+    // final scale applies it
+    bool folded = false;
+    for (const auto& t : s.ops) folded = folded || (t.fold != FOLD_NONE);
+    if (folded) {
+      bool absorbed = false;
+      for (auto& t : s.ops)
+        if (t.code == TC_U1 && t.fold == FOLD_NONE && t.b == U1_GENERAL && t.ns == 0 && t.lane_cmask == 0) {
+          t.fold = FOLD_ABSORB;
+          absorbed = true;
+          break;
+        }
+      if (!absorbed) {
+        TileOpHost t{};
+        t.code = TC_SCALE;
+        t.fold = FOLD_SCALE;
+        s.ops.push_back(t);
+      }
+    }
     // ---- final layout: low tile positions on the low thread slots for a coalesced store --
     bool canon = true;
     for (int sl = 0; sl < c; ++sl)
@@ -456,10 +494,57 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         if (__builtin_popcountll(Rf) < r) Rf |= 1ull << b;
       emit_remap(make_map(Rf));
     }
+    {  // plan-time values of the coefficients (the CPU model reads them)
+      std::vector<cd> c;
+      stage_coefficients(s, ops, c);
+      for (size_t i = 0; i < s.ops.size(); ++i)
+        for (int q = 0; q < 4; ++q) s.ops[i].coef[q] = c[4 * i + q];
+    }
     stages.push_back(std::move(s));
     remaining = left;
   }
   return stages;
+}
+
+void stage_coefficients(const Stage& s, const std::vector<FastOp>& ops, std::vector<cd>& c) {
+  c.assign(4 * s.ops.size(), cd(0, 0));
+  cd total(1, 0);
+  int absorber = -1, scaler = -1;
+  for (size_t i = 0; i < s.ops.size(); ++i) {
+    const TileOpHost& t = s.ops[i];
+    cd* o = &c[4 * i];
+    if (t.src < 0) {
+      for (int q = 0; q < 4; ++q) o[q] = t.coef[q];
+      if (t.code == TC_GRAD) o[0] = cd((double)t.param, 0.0);  // the tap's row
+      if (t.fold == FOLD_SCALE) scaler = (int)i;
+      continue;
+    }
+    const std::vector<cd>& m = ops[t.src].m;
+    for (size_t q = 0; q < 4 && q < m.size(); ++q) o[q] = m[q];
+    cd f(1, 0);
+    switch (t.fold) {
+      case FOLD_M00: f = m[0]; break;
+      case FOLD_M10: f = m[2]; break;
+      case FOLD_D0: f = m[0]; break;
+      case FOLD_ABSORB: absorber = (int)i; break;
+      default: break;
+    }
+    if (t.fold == FOLD_M00 || t.fold == FOLD_M10) {  // the residual matrix is exact
+      static const cd had[4] = {cd(1, 0), cd(1, 0), cd(1, 0), cd(-1, 0)};
+      static const cd xs[4] = {cd(0, 0), cd(1, 0), cd(1, 0), cd(0, 0)};
+      static const cd ns[4] = {cd(0, 0), cd(-1, 0), cd(1, 0), cd(0, 0)};
+      const cd* r = t.b == TF_HAD ? had : (t.b == TF_X ? xs : ns);
+      for (int q = 0; q < 4; ++q) o[q] = r[q];
+      total *= f;
+    } else if (t.fold == FOLD_D0) {
+      o[1] = m[1] / f;
+      o[0] = cd(1, 0);
+      total *= f;
+    }
+  }
+  if (absorber >= 0)
+    for (int q = 0; q < 4; ++q) c[4 * absorber + q] *= total;
+  if (scaler >= 0) c[4 * scaler] = total;
 }
 
 }  // namespace qr

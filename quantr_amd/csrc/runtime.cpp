@@ -68,17 +68,12 @@ std::vector<uint8_t> pack_blob(const std::vector<FastOp>& ops, const std::vector
         t.map = maps_op ? h.a : 0;
         t.swz = h.swz;
         d[i] = t;
-        if (h.src >= 0) {  // the current program's coefficients (plans are shared by structure)
-          const std::vector<cd>& m = ops[h.src].m;
-          for (int q = 0; q < 4; ++q)
-            c[4 * i + q] = q < (int)m.size() ? make_double2(m[q].real(), m[q].imag()) : make_double2(0.0, 0.0);
-        } else if (h.code == TC_GRAD) {  // parameter row of the tap (an exact small integer)
-          c[4 * i] = make_double2((double)h.param, 0.0);
-          for (int q = 1; q < 4; ++q) c[4 * i + q] = make_double2(0.0, 0.0);
-        } else {
-          for (int q = 0; q < 4; ++q) c[4 * i + q] = make_double2(h.coef[q].real(), h.coef[q].imag());
-        }
       }
+      // the current program's coefficients (plans are shared by structure), folded scalars
+      // applied; gradient taps carry their parameter row
+      std::vector<cd> cv;
+      stage_coefficients(s, ops, cv);
+      for (size_t i = 0; i < cv.size(); ++i) c[i] = make_double2(cv[i].real(), cv[i].imag());
       loc[si].ops = append(d.data(), d.size() * sizeof(TileOpDev));
       align(16);
       loc[si].coefs = append(c.data(), c.size() * sizeof(double2));

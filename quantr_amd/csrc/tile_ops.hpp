@@ -63,6 +63,15 @@ __device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const Coef4 u, uint32_t 
     R o0r, o0i, o1r, o1i;
     if (FORM == 3) {  // X: swap
       o0r = yr; o0i = yi; o1r = xr; o1i = xi;
+    } else if (FORM == 4) {  // real diagonal, imaginary off-diagonal (Rx)
+      o0r = fma(-u01i, yi, u00r * xr);
+      o0i = fma(u01i, yr, u00r * xi);
+      o1r = fma(-u10i, xi, u11r * yr);
+      o1i = fma(u10i, xr, u11r * yi);
+    } else if (FORM == 5) {  // [[1,1],[1,-1]]
+      o0r = xr + yr; o0i = xi + yi; o1r = xr - yr; o1i = xi - yi;
+    } else if (FORM == 6) {  // [[0,-1],[1,0]]
+      o0r = -yr; o0i = -yi; o1r = xr; o1i = xi;
     } else if (FORM == 1) {  // anti-diagonal
       o0r = fma(u01r, yr, -u01i * yi);
       o0i = fma(u01r, yi, u01i * yr);
@@ -88,12 +97,32 @@ __device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const Coef4 u, uint32_t 
   }
 }
 
+// X on register slot J in the lanes where `ok` holds (controls held by threads): a per-lane
+// select of the two amplitudes (compiled stages, where J is a constant).
+template <typename R, int RB, int J, bool HAS_NS>
+__device__ __forceinline__ void u1_xsel(Regs<R, RB>& a, bool ok, uint32_t ns) {
+#pragma unroll
+  for (int m0 = 0; m0 < (1 << RB); ++m0) {
+    if (m0 & (1 << J)) continue;
+    if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
+    const int m1 = m0 | (1 << J);
+    const typename CT<R>::T x = a.v[m0], y = a.v[m1];
+    a.v[m0].x = ok ? y.x : x.x;
+    a.v[m0].y = ok ? y.y : x.y;
+    a.v[m1].x = ok ? x.x : y.x;
+    a.v[m1].y = ok ? x.y : y.y;
+  }
+}
+
 template <typename R, int RB, int J, bool HAS_NS>
 __device__ __forceinline__ void u1_form(Regs<R, RB>& a, const Coef4 u, uint32_t form, uint32_t ns) {
   switch (form) {
     case 3: u1_slot<R, RB, J, 3, HAS_NS>(a, u, ns); break;
     case 1: u1_slot<R, RB, J, 1, HAS_NS>(a, u, ns); break;
     case 2: u1_slot<R, RB, J, 2, HAS_NS>(a, u, ns); break;
+    case 4: u1_slot<R, RB, J, 4, HAS_NS>(a, u, ns); break;
+    case 5: u1_slot<R, RB, J, 5, HAS_NS>(a, u, ns); break;
+    case 6: u1_slot<R, RB, J, 6, HAS_NS>(a, u, ns); break;
     default: u1_slot<R, RB, J, 0, HAS_NS>(a, u, ns); break;
   }
 }

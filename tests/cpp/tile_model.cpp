@@ -147,6 +147,11 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
           }
         }
       }
+      if (op.code == TC_SCALE) {
+        for (auto* rs : sets)
+          for (auto& th : *rs)
+            for (auto& v : th) v *= op.coef[0];
+      }
       if (op.code == TC_REMAP) {
         const uint16_t* P = &s.swz[(size_t)op.swz * 2 * K];
         auto lofs = [&](int which, int t, int m) {
