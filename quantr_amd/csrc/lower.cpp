@@ -407,8 +407,9 @@ void classify(FastOp& f) {
   } else if (A[0].imag() == 0 && A[1].imag() == 0 && A[2].imag() == 0 && A[3].imag() == 0) {
     f.form = U1_REAL;
     if (A[0] == A[1] && A[0] == A[2] && A[3] == -A[0]) f.sub = SUB_HAD;
+    else if (A[0] == A[3] && A[1] == -A[2]) f.sub = SUB_RROT;
   } else if (A[0].imag() == 0 && A[3].imag() == 0 && A[1].real() == 0 && A[2].real() == 0) {
-    f.sub = SUB_RX;
+    f.sub = (A[0] == A[3] && A[1] == A[2]) ? SUB_XROT : SUB_RX;
   }
 }
 

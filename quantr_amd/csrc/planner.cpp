@@ -12,6 +12,7 @@
 // amplitudes through LDS so other tile bits become register bits. Diagonal gates and
 // controls may sit on any bit, in or out of the tile.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "engine.hpp"
@@ -399,7 +400,13 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           const bool free = !dual && t.ns == 0 && t.lane_cmask == 0;  // uncontrolled: scalar is global
           if (x) {
             form = TF_X;
-          } else if (op.sub == SUB_RX) {
+          } else if (free && op.sub == SUB_RROT) {
+            form = TF_RROT;
+            t.fold = FOLD_RROT;
+          } else if (free && op.sub == SUB_XROT) {
+            form = TF_XROT;
+            t.fold = FOLD_XROT;
+          } else if (op.sub == SUB_RX || op.sub == SUB_XROT) {
             form = TF_RX;
           } else if (free && op.sub == SUB_HAD) {
             form = TF_HAD;
@@ -540,6 +547,28 @@ void stage_coefficients(const Stage& s, const std::vector<FastOp>& ops, std::vec
       o[1] = m[1] / f;
       o[0] = cd(1, 0);
       total *= f;
+    } else if (t.fold == FOLD_RROT) {  // [[a,-b],[b,a]]
+      const double a = m[0].real(), b = m[2].real();
+      if (std::fabs(a) >= std::fabs(b)) {
+        const double tq = b / a;
+        o[0] = cd(1, 0); o[1] = cd(-tq, 0); o[2] = cd(tq, 0); o[3] = cd(1, 0);
+        total *= a;
+      } else {
+        const double u = a / b;
+        o[0] = cd(u, 0); o[1] = cd(-1, 0); o[2] = cd(1, 0); o[3] = cd(u, 0);
+        total *= b;
+      }
+    } else if (t.fold == FOLD_XROT) {  // [[a,-ib],[-ib,a]]
+      const double a = m[0].real(), b = -m[1].imag();
+      if (std::fabs(a) >= std::fabs(b)) {
+        const double tq = b / a;
+        o[0] = cd(1, 0); o[1] = cd(0, -tq); o[2] = cd(0, -tq); o[3] = cd(1, 0);
+        total *= a;
+      } else {
+        const double u = a / b;
+        o[0] = cd(0, u); o[1] = cd(1, 0); o[2] = cd(1, 0); o[3] = cd(0, u);
+        total *= cd(0, -b);
+      }
     }
   }
   if (absorber >= 0)
