@@ -82,10 +82,11 @@ def bench_simulate(args):
                    reg_bits=args.reg_bits, jit=args.jit)
     log("bench: %d-qubit %s brickwork depth %d, %d gates" % (n, args.precision, args.depth, len(gates)))
     for i in range(args.warmup):
+        t0 = time.perf_counter()
         st.reset()
         st.apply(prog)
         st.synchronize()
-        log("bench: warmup step %d done" % i)
+        log("bench: warmup step %d done in %.2f s (first use compiles the stages)" % (i, time.perf_counter() - t0))
     plan = st.last_plan()
     st.profile_enable(True)
     st.synchronize()

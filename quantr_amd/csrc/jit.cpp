@@ -10,8 +10,17 @@
 // parameter-shifted variants). Modules are cached per process, keyed by their source text.
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
+#include <spawn.h>
+#include <sys/stat.h>
+#include <sys/wait.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <cerrno>
+#include <cstdlib>
+#include <fcntl.h>
+#include <cstring>
+#include <fstream>
 #include <cstdio>
 #include <map>
 #include <memory>
@@ -28,6 +37,11 @@
 namespace qr {
 
 extern const char* const kTileOpsSource;  // text of tile_ops.hpp, embedded at build time
+}  // namespace qr
+
+extern char** environ;
+
+namespace qr {
 
 namespace {
 
@@ -307,6 +321,94 @@ std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int
   return out;
 }
 
+namespace {
+
+// ---- parallel compilation in child processes ----------------------------------------------
+// hipRTC compiles one program at a time per process, so a plan of tens of stages would compile
+// serially. Stages are instead split into groups, each written as one source with uniquely
+// named kernels and compiled by the HIP compiler in its own child process (all groups at
+// once); every group becomes one module. hipRTC in this process is the fallback.
+
+std::string compiler_path() {
+  const char* rp = std::getenv("ROCM_PATH");
+  std::string p = std::string(rp && *rp ? rp : "/opt/rocm") + "/lib/llvm/bin/clang++";
+  return access(p.c_str(), X_OK) == 0 ? p : std::string();
+}
+
+std::string renamed(const std::string& kernel, const std::string& name) {
+  std::string k = kernel;
+  const size_t at = k.find("void qr_stage(");
+  if (at != std::string::npos) k.replace(at, 14, "void " + name + "(");
+  return k;
+}
+
+bool read_file(const std::string& path, std::vector<char>& out) {
+  std::ifstream f(path, std::ios::binary);
+  if (!f) return false;
+  out.assign(std::istreambuf_iterator<char>(f), std::istreambuf_iterator<char>());
+  return !out.empty();
+}
+
+struct Group {
+  std::vector<size_t> items;  // indices into the todo list
+  std::string src, obj, log;
+  pid_t pid = -1;
+  bool ok = false;
+};
+
+// Compiles todo[i] for every i of every group; returns the groups that produced a code object.
+void compile_in_children(const std::vector<std::string>& todo, std::vector<Group>& groups, const std::string& cc) {
+  char tmpl[] = "/tmp/qr_stages_XXXXXX";
+  const char* td = std::getenv("TMPDIR");
+  std::string base = std::string(td && *td ? td : "/tmp") + "/qr_stages_XXXXXX";
+  std::vector<char> dir(base.begin(), base.end());
+  dir.push_back('\0');
+  if (!mkdtemp(dir.data())) {
+    std::memcpy(dir.data(), tmpl, sizeof tmpl);
+    dir.resize(sizeof tmpl);
+    if (!mkdtemp(dir.data())) return;
+  }
+  const std::string d(dir.data());
+  for (size_t g = 0; g < groups.size(); ++g) {
+    Group& G = groups[g];
+    G.src = d + "/g" + std::to_string(g) + ".hip";
+    G.obj = d + "/g" + std::to_string(g) + ".co";
+    G.log = d + "/g" + std::to_string(g) + ".log";
+    std::ofstream f(G.src);
+    f << "#include <hip/hip_runtime.h>\n" << kTileOpsSource << "\n";
+    for (size_t j = 0; j < G.items.size(); ++j) f << renamed(todo[G.items[j]], "qr_stage_" + std::to_string(j));
+    f.close();
+    if (!f) continue;
+    std::vector<std::string> args = {cc, "-x", "hip", "--offload-arch=gfx950", "--cuda-device-only",
+                                     "--no-gpu-bundle-output", "-O3", "-std=c++17", "-Xclang", "-target-feature",
+                                     "-Xclang", "-packed-fp32-ops", "-Wno-pragma-once-outside-header", "-w",
+                                     "-o", G.obj, G.src};
+    std::vector<char*> argv;
+    for (auto& a : args) argv.push_back(&a[0]);
+    argv.push_back(nullptr);
+    posix_spawn_file_actions_t fa;
+    posix_spawn_file_actions_init(&fa);
+    posix_spawn_file_actions_addopen(&fa, 1, G.log.c_str(), O_WRONLY | O_CREAT | O_TRUNC, 0644);
+    posix_spawn_file_actions_adddup2(&fa, 1, 2);
+    if (posix_spawn(&G.pid, cc.c_str(), &fa, nullptr, argv.data(), environ) != 0) G.pid = -1;
+    posix_spawn_file_actions_destroy(&fa);
+  }
+  for (auto& G : groups) {
+    if (G.pid <= 0) continue;
+    int st = 0;
+    while (waitpid(G.pid, &st, 0) < 0 && errno == EINTR) {
+    }
+    G.ok = WIFEXITED(st) && WEXITSTATUS(st) == 0;
+  }
+  for (auto& G : groups) {
+    (void)unlink(G.src.c_str());
+    (void)unlink(G.log.c_str());
+  }
+  (void)d;
+}
+
+}  // namespace
+
 void jit_prepare(const std::vector<std::string>& sources) {
   std::vector<std::string> todo;
   {
@@ -315,36 +417,54 @@ void jit_prepare(const std::vector<std::string>& sources) {
       if (!s.empty() && !g_cache.count(s) && std::find(todo.begin(), todo.end(), s) == todo.end()) todo.push_back(s);
   }
   if (todo.empty()) return;
-  std::vector<std::vector<char>> codes(todo.size());
-  std::vector<std::string> errs(todo.size());
-  std::vector<char> ok(todo.size(), 0);
-  const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-  size_t next = 0;
-  std::mutex qm;
-  auto worker = [&]() {
-    while (true) {
-      size_t i;
-      {
-        std::lock_guard<std::mutex> lk(qm);
-        if (next >= todo.size()) return;
-        i = next++;
+  std::vector<char> done(todo.size(), 0);
+  std::vector<JitKernel> made(todo.size());
+  const std::string cc = todo.size() > 1 ? compiler_path() : std::string();
+  if (!cc.empty()) {
+    const unsigned hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    const size_t ng = std::min<size_t>(hw, todo.size());
+    std::vector<Group> groups(ng);
+    for (size_t i = 0; i < todo.size(); ++i) groups[i * ng / todo.size()].items.push_back(i);
+    compile_in_children(todo, groups, cc);
+    for (auto& G : groups) {
+      std::vector<char> code;
+      if (G.ok && read_file(G.obj, code)) {
+        hipModule_t mod = nullptr;
+        if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
+          for (size_t j = 0; j < G.items.size(); ++j) {
+            hipFunction_t fn = nullptr;
+            if (hipModuleGetFunction(&fn, mod, ("qr_stage_" + std::to_string(j)).c_str()) == hipSuccess) {
+              made[G.items[j]] = JitKernel{mod, fn};
+              done[G.items[j]] = 1;
+            }
+          }
+        }
       }
-      ok[i] = compile_one(full_source(todo[i]), codes[i], errs[i]) ? 1 : 0;
+      (void)unlink(G.obj.c_str());
     }
-  };
-  std::vector<std::thread> th;
-  for (unsigned t = 0; t < std::min<size_t>(hw, todo.size()); ++t) th.emplace_back(worker);
-  for (auto& t : th) t.join();
-  std::lock_guard<std::mutex> lk(g_mu);
+    if (!groups.empty()) {
+      const std::string dir = groups[0].obj.substr(0, groups[0].obj.rfind('/'));
+      if (!dir.empty()) (void)rmdir(dir.c_str());
+    }
+  }
+  // fallback: hipRTC in this process
   for (size_t i = 0; i < todo.size(); ++i) {
-    if (!ok[i]) throw Error{QR_ERR_HIP, "runtime compilation of a tile stage failed: " + errs[i]};
+    if (done[i]) continue;
+    std::vector<char> code;
+    std::string err;
+    if (!compile_one(full_source(todo[i]), code, err))
+      throw Error{QR_ERR_HIP, "runtime compilation of a tile stage failed: " + err};
     JitKernel k;
-    hipError_t e = hipModuleLoadData(&k.mod, codes[i].data());
+    hipError_t e = hipModuleLoadData(&k.mod, code.data());
     if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e)};
     e = hipModuleGetFunction(&k.fn, k.mod, "qr_stage");
     if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleGetFunction: ") + hipGetErrorString(e)};
-    g_cache[todo[i]] = k;
+    made[i] = k;
+    done[i] = 1;
   }
+  std::lock_guard<std::mutex> lk(g_mu);
+  for (size_t i = 0; i < todo.size(); ++i)
+    if (!g_cache.count(todo[i])) g_cache[todo[i]] = made[i];
 }
 
 hipFunction_t jit_function(const std::string& source) {
