@@ -128,6 +128,18 @@ def bench_adjoint(args):
         grads = g.backward()
     t_adj = (time.perf_counter() - t0) / reps
     g.close()
+    # the same in complex64, against the complex128 gradient
+    g64 = _capi.Grad(n, _capi.QR_C64)
+    for _ in range(2):
+        g64.forward(prog, obs)
+        g64.backward()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g64.forward(prog, obs)
+        grads64 = g64.backward()
+    t_adj64 = (time.perf_counter() - t0) / reps
+    g64.close()
+    err64 = float(np.max(np.abs(grads64 - grads)) / max(1e-300, float(np.max(np.abs(grads)))))
     # parameter shift: two plain simulations + expectation per parameter; the shifted
     # programs are built before the clock starts, so only device work and planning count
     st = _capi.State(n, _capi.QR_C128)
@@ -156,7 +168,9 @@ def bench_adjoint(args):
     st.close()
     err = float(np.max(np.abs(grads[:m] - shift)) / max(1e-300, float(np.max(np.abs(shift)))))
     log("bench: adjoint %.2f ms for %d grads; parameter shift %.1f ms; rel err %.2e" % (t_adj * 1e3, P, t_ps * 1e3, err))
+    log("bench: adjoint complex64 %.2f ms for %d grads, rel err vs complex128 %.2e" % (t_adj64 * 1e3, P, err64))
     return {"adjoint_grads_per_s": P / t_adj, "param_shift_grads_per_s": P / t_ps,
+            "adjoint_c64_grads_per_s": P / t_adj64, "adjoint_c64_rel_err_vs_c128": err64,
             "adjoint_vs_param_shift": t_ps / t_adj, "adjoint_rel_err_vs_shift": err, "energy": e,
             "params": P, "qubits": n, "layers": L}
 
@@ -259,10 +273,10 @@ def bench_batch(args):
         for m, L, g, _ in inst:
             if m == n and L not in reps:
                 reps[L] = (m, W.to_program(g), obs[m])
-        warm = _capi.Batch(list(reps.values()), _capi.QR_C64)
+        warm = _capi.Batch(list(reps.values()), _capi.QR_C64, autotune=True)
         warm.close()
         log("bench: batch stages compiled for %d qubits (%d layer counts), %.1f s" % (n, len(reps), time.perf_counter() - t0))
-    b = _capi.Batch(items, _capi.QR_C64)
+    b = _capi.Batch(items, _capi.QR_C64, autotune=True)
     setup = time.perf_counter() - t0
     b.run()  # warm-up
     reps = 3
@@ -313,6 +327,7 @@ def bench_batch(args):
             s_.close()
     lat_sorted = np.sort(lat)
     out = {"instances": len(inst), "params": P, "precision": "c64", "setup_s": round(setup, 2),
+           "setup": "planning, tile-width autotuning (qubit counts >= 16), stage compilation, graph capture",
            "run_ms": round(t_run * 1e3, 3), "grads_per_s": round(P / t_run, 1),
            "latency_us_p50": round(float(np.percentile(lat_sorted, 50)), 1),
            "latency_us_p99": round(float(np.percentile(lat_sorted, 99)), 1),

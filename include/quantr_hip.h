@@ -238,9 +238,23 @@ qr_status qr_grad_set_options(qr_grad* grad, const qr_options* options);
  * Runs may pass new programs of the same structure (new angles, new custom tables of the same
  * shape); NULL reuses the programs given at creation. Observables are fixed at creation. */
 typedef struct qr_batch qr_batch;
+/* Batch options (NULL = defaults: all zero).
+ *  autotune        1: for every qubit count >= 16 in the batch, time the candidate tile widths
+ *                  ("fusion widths") of the reverse sweep on one representative instance and
+ *                  keep the fastest (cached per process); 0: built-in widths
+ *  fwd_tile_bits   forward-sweep tile width (0 = auto)
+ *  dual_tile_bits  reverse-sweep tile width (0 = auto or tuned)
+ *  streams         concurrent streams inside the graph (0 = 8) */
+typedef struct qr_batch_options {
+  uint32_t autotune;
+  uint32_t fwd_tile_bits;
+  uint32_t dual_tile_bits;
+  uint32_t streams;
+} qr_batch_options;
 qr_status qr_batch_create(uint64_t ninstances, const uint32_t* num_qubits,
                           const qr_program* programs, const qr_observable* observables,
-                          qr_precision precision, qr_batch** out);
+                          qr_precision precision, const qr_batch_options* options,
+                          qr_batch** out);
 qr_status qr_batch_destroy(qr_batch* batch);
 /* Parameters (Rx/Ry/Rz/Phase gates) of each instance: counts_out[ninstances]. */
 qr_status qr_batch_param_counts(const qr_batch* batch, uint64_t* counts_out);

@@ -47,6 +47,11 @@ class QrOptions(C.Structure):
                 ("jit", C.c_uint32)]
 
 
+class QrBatchOptions(C.Structure):
+    _fields_ = [("autotune", C.c_uint32), ("fwd_tile_bits", C.c_uint32), ("dual_tile_bits", C.c_uint32),
+                ("streams", C.c_uint32)]
+
+
 class QrPlanStats(C.Structure):
     _fields_ = [("gates", C.c_uint64), ("sweeps", C.c_uint64), ("tile_sweeps", C.c_uint64),
                 ("bytes", C.c_double)]
@@ -87,7 +92,7 @@ _SIGNATURES = [
     ("qr_grad_backward", C.c_int, [C.c_void_p, C.c_double, C.c_void_p, C.c_uint64]),
     ("qr_grad_set_options", C.c_int, [C.c_void_p, C.c_void_p]),
     ("qr_batch_create", C.c_int, [C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int,
-                                  C.POINTER(C.c_void_p)]),
+                                  C.c_void_p, C.POINTER(C.c_void_p)]),
     ("qr_batch_destroy", C.c_int, [C.c_void_p]),
     ("qr_batch_param_counts", C.c_int, [C.c_void_p, C.c_void_p]),
     ("qr_batch_run", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
@@ -385,7 +390,7 @@ class Batch:
     """Many (num_qubits, program, observable) instances, each with expectation value and full
     adjoint gradient, executed as one captured graph launch per run (qr_batch)."""
 
-    def __init__(self, instances, precision=QR_C64):
+    def __init__(self, instances, precision=QR_C64, autotune=False, fwd_tile_bits=0, dual_tile_bits=0, streams=0):
         self.m = len(instances)
         self.n = [int(i[0]) for i in instances]
         nq = np.array(self.n, dtype=np.uint32)
@@ -397,8 +402,9 @@ class Batch:
             obs[i] = os_
         progs, keep = self._programs([i[1] for i in instances])
         self._h = C.c_void_p()
+        opts = QrBatchOptions(1 if autotune else 0, fwd_tile_bits, dual_tile_bits, streams)
         check(lib().qr_batch_create(self.m, nq.ctypes.data, C.addressof(progs), C.addressof(obs), precision,
-                                    C.byref(self._h)))
+                                    C.byref(opts), C.byref(self._h)))
         self.counts = np.zeros(self.m, dtype=np.uint64)
         check(lib().qr_batch_param_counts(self._h, self.counts.ctypes.data))
         self.offsets = np.zeros(self.m, dtype=np.uint64)

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <mutex>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -88,18 +90,23 @@ namespace qr {
 
 namespace {
 
-PlanParams fwd_params(uint32_t n, int prec) {
+// Tile widths of one instance (0 = the default for its precision).
+struct Widths {
+  int fwd = 0, dual = 0;
+};
+
+PlanParams fwd_params(uint32_t n, int prec, int k = 0) {
   PlanParams pp;
-  pp.tile_bits = std::min<int>(prec == 0 ? 12 : 13, (int)n);
+  pp.tile_bits = std::min<int>(k ? k : (prec == 0 ? 12 : 13), (int)n);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
   pp.low_bits = prec == 0 ? 3 : 4;
   if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
   return pp;
 }
 
-PlanParams dual_params(uint32_t n, int prec) {
+PlanParams dual_params(uint32_t n, int prec, int k = 0) {
   PlanParams pp;
-  pp.tile_bits = std::min<int>(prec == 0 ? 10 : 11, (int)n);
+  pp.tile_bits = std::min<int>(k ? k : (prec == 0 ? 10 : 11), (int)n);
   pp.reg_bits = std::min(4, pp.tile_bits);
   pp.low_bits = prec == 0 ? 3 : 4;
   if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
@@ -209,10 +216,37 @@ void launch_instance(qr_batch* b, const Inst& I, hipStream_t s, uint64_t* stamps
   hip_check(launch_stamp(stamps + 1 + idx, s), "stamp");
 }
 
+
+qr_batch* batch_create_widths(uint64_t m, const uint32_t* nq, const qr_program* programs,
+                              const qr_observable* observables, qr_precision precision,
+                              const std::vector<Widths>& widths, int nstreams);
+void autotune_widths(uint64_t m, const uint32_t* nq, const qr_program* programs, const qr_observable* observables,
+                     qr_precision precision, std::vector<Widths>& widths);
+
 }  // namespace
 
 qr_batch* batch_create(uint64_t m, const uint32_t* nq, const qr_program* programs, const qr_observable* observables,
-                       qr_precision precision) {
+                       qr_precision precision, const qr_batch_options* o) {
+  if (!m) throw Error{QR_ERR_ARG, "a batch needs at least one instance"};
+  if (!nq || !programs || !observables) throw Error{QR_ERR_ARG, "NULL argument"};
+  const int maxk = tile_max_bits(precision == QR_C128 ? 0 : 1);
+  if (o && ((int)o->fwd_tile_bits > maxk || (int)o->dual_tile_bits > maxk))
+    throw Error{QR_ERR_ARG, "tile width above " + std::to_string(maxk)};
+  std::vector<Widths> widths(m);
+  for (auto& w : widths) {
+    w.fwd = o ? (int)o->fwd_tile_bits : 0;
+    w.dual = o ? (int)o->dual_tile_bits : 0;
+  }
+  const int nstreams = (o && o->streams) ? (int)std::min<uint32_t>(o->streams, 32) : 8;
+  if (o && o->autotune && !o->dual_tile_bits) autotune_widths(m, nq, programs, observables, precision, widths);
+  return batch_create_widths(m, nq, programs, observables, precision, widths, nstreams);
+}
+
+namespace {
+
+qr_batch* batch_create_widths(uint64_t m, const uint32_t* nq, const qr_program* programs,
+                              const qr_observable* observables, qr_precision precision,
+                              const std::vector<Widths>& widths, int nstreams) {
   if (!m) throw Error{QR_ERR_ARG, "a batch needs at least one instance"};
   if (!nq || !programs || !observables) throw Error{QR_ERR_ARG, "NULL argument"};
   if (precision != QR_C128 && precision != QR_C64) throw Error{QR_ERR_ARG, "unknown precision"};
@@ -231,8 +265,8 @@ qr_batch* batch_create(uint64_t m, const uint32_t* nq, const qr_program* program
     lower_instance(I, &programs[i]);
     I.key = structure_key(I.fwd);
     I.terms = parse_observable(I.n, &observables[i]);
-    I.ppf = fwd_params(I.n, b->prec);
-    I.ppb = dual_params(I.n, b->prec);
+    I.ppf = fwd_params(I.n, b->prec, widths[i].fwd);
+    I.ppb = dual_params(I.n, b->prec, widths[i].dual);
     I.pf = get_plan_for(I.n, b->prec, I.fwd, I.ppf, true, false, false);
     I.pb = get_plan_for(I.n, b->prec, I.rev, I.ppb, true, true, false);
     for (const auto& op : I.fwd)
@@ -329,7 +363,7 @@ qr_batch* batch_create(uint64_t m, const uint32_t* nq, const qr_program* program
   b->work.reserve(std::max<size_t>(work_doubles, 1) * sizeof(double));
   b->out.reserve(out_doubles * sizeof(double) + (m + 1) * sizeof(uint64_t) + kAlign);
   // ---- streams: largest instances first, each to the least loaded stream ---------------------
-  const int S = (int)std::min<uint64_t>(8, m);
+  const int S = (int)std::min<uint64_t>((uint64_t)nstreams, m);
   hip_check(hipStreamCreateWithFlags(&b->origin, hipStreamNonBlocking), "hipStreamCreate");
   b->streams.resize(S);
   b->tmp.resize(S);
@@ -380,6 +414,8 @@ qr_batch* batch_create(uint64_t m, const uint32_t* nq, const qr_program* program
   return b.release();
 }
 
+}  // namespace
+
 void batch_run(qr_batch* b, const qr_program* programs, double* expvals, double* grads, const uint64_t* grad_offsets,
                double* latency_us) {
   const size_t m = b->inst.size();
@@ -425,5 +461,65 @@ void batch_param_counts(const qr_batch* b, uint64_t* out) {
 }
 
 void batch_destroy(qr_batch* b) { delete b; }
+
+namespace {
+
+std::mutex g_tune_mu;
+std::map<std::pair<uint32_t, int>, int> g_tuned;  // (qubits, precision) -> reverse-sweep tile width
+
+// For every qubit count >= 16: the reverse-sweep tile widths around the default, timed on the
+// instance with the longest program (one-instance batches, best of three runs).
+void autotune_widths(uint64_t m, const uint32_t* nq, const qr_program* programs, const qr_observable* observables,
+                     qr_precision precision, std::vector<Widths>& widths) {
+  const int prec = precision == QR_C128 ? 0 : 1;
+  const int def = prec == 0 ? 10 : 11;
+  std::map<uint32_t, uint64_t> rep;
+  for (uint64_t i = 0; i < m; ++i)
+    if (nq[i] >= 16 && (!rep.count(nq[i]) || programs[i].nops > programs[rep[nq[i]]].nops)) rep[nq[i]] = i;
+  for (const auto& kv : rep) {
+    const uint32_t n = kv.first;
+    const uint64_t i = kv.second;
+    int best = def;
+    bool cached = false;
+    {
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      auto it = g_tuned.find({n, prec});
+      if (it != g_tuned.end()) {
+        best = it->second;
+        cached = true;
+      }
+    }
+    if (!cached) {
+      double best_t = 1e300;
+      for (int k : {def - 1, def, def + 1}) {
+        if (k < 6 || k > tile_max_bits(prec) || k > (int)n) continue;
+        std::vector<Widths> w(1);
+        w[0].fwd = widths[i].fwd;
+        w[0].dual = k;
+        std::unique_ptr<qr_batch, void (*)(qr_batch*)> b(
+            batch_create_widths(1, &nq[i], &programs[i], &observables[i], precision, w, 1), batch_destroy);
+        std::vector<double> g(std::max<uint64_t>(b->total_params, 1));
+        double e = 0.0;
+        batch_run(b.get(), nullptr, &e, g.data(), nullptr, nullptr);  // warm-up
+        double t = 1e300;
+        for (int r = 0; r < 3; ++r) {
+          const auto t0 = std::chrono::steady_clock::now();
+          batch_run(b.get(), nullptr, &e, g.data(), nullptr, nullptr);
+          t = std::min(t, std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count());
+        }
+        if (t < best_t) {
+          best_t = t;
+          best = k;
+        }
+      }
+      std::lock_guard<std::mutex> lk(g_tune_mu);
+      g_tuned[{n, prec}] = best;
+    }
+    for (uint64_t j = 0; j < m; ++j)
+      if (nq[j] == n) widths[j].dual = best;
+  }
+}
+
+}  // namespace
 
 }  // namespace qr

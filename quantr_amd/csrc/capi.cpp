@@ -426,11 +426,12 @@ qr_status qr_grad_backward(qr_grad* g, double grad_out, double* grads_out, uint6
 }
 
 qr_status qr_batch_create(uint64_t ninstances, const uint32_t* num_qubits, const qr_program* programs,
-                          const qr_observable* observables, qr_precision precision, qr_batch** out) {
+                          const qr_observable* observables, qr_precision precision, const qr_batch_options* options,
+                          qr_batch** out) {
   return guard([&] {
     if (!out) throw Error{QR_ERR_ARG, "out is NULL"};
     *out = nullptr;
-    *out = batch_create(ninstances, num_qubits, programs, observables, precision);
+    *out = batch_create(ninstances, num_qubits, programs, observables, precision, options);
   });
 }
 
@@ -459,7 +460,8 @@ qr_status qr_batch_grad(uint64_t ninstances, const uint32_t* num_qubits, const q
   return guard([&] {
     if (!ninstances) return;
     if (!expvals_out) throw Error{QR_ERR_ARG, "NULL argument"};
-    std::unique_ptr<qr_batch, void (*)(qr_batch*)> b(batch_create(ninstances, num_qubits, programs, observables, precision),
+    std::unique_ptr<qr_batch, void (*)(qr_batch*)> b(batch_create(ninstances, num_qubits, programs, observables, precision,
+                                                                  nullptr),
                                                      [](qr_batch* x) { batch_destroy(x); });
     batch_run(b.get(), nullptr, expvals_out, grads_out, grad_offsets, latency_us_out);
   });

@@ -175,7 +175,7 @@ void backward_fused(qr_grad* g, double grad_out, double* grads_out);
 
 // ---- batches (batch.cpp) ------------------------------------------------------------------------
 qr_batch* batch_create(uint64_t m, const uint32_t* nq, const qr_program* programs, const qr_observable* observables,
-                       qr_precision precision);
+                       qr_precision precision, const qr_batch_options* options);
 void batch_run(qr_batch* b, const qr_program* programs, double* expvals, double* grads, const uint64_t* grad_offsets,
                double* latency_us);
 void batch_param_counts(const qr_batch* b, uint64_t* out);

@@ -435,3 +435,24 @@ def test_batch_rejects_structure_change():
     with pytest.raises(_capi.QrError):
         b.run([W.to_program(gates[:-1])])
     b.close()
+
+
+def test_batch_autotuned_widths_match():
+    """Tuned reverse-sweep tile widths change the plan, not the results."""
+    rng = np.random.default_rng(911)
+    items = []
+    for n in (16, 17, 12):
+        gates, _ = W.hea(n, 2, seed=int(rng.integers(1 << 30)))
+        items.append((n, W.to_program(gates), _capi.Observable(W.hea_observable(n))))
+    say("batch autotune")
+    b0 = _capi.Batch(items, _capi.QR_C128)
+    b1 = _capi.Batch(items, _capi.QR_C128, autotune=True)
+    b2 = _capi.Batch(items, _capi.QR_C128, dual_tile_bits=9, fwd_tile_bits=11)
+    e0, g0, _ = b0.run()
+    for b in (b1, b2):
+        e, g, _ = b.run()
+        assert np.max(np.abs(e - e0)) < 1e-11
+        for x, y in zip(g, g0):
+            assert np.max(np.abs(x - y)) < 1e-11
+        b.close()
+    b0.close()
