@@ -13,6 +13,7 @@
 #pragma once
 
 #include <algorithm>
+#include <charconv>
 #include <complex>
 #include <cstdint>
 #include <cstdio>
@@ -48,6 +49,19 @@ class QuantrError : public std::runtime_error {
 namespace detail {
 inline void check(qr_status s) {
   if (s != QR_OK) throw QuantrError(qr_last_error());
+}
+// Rust's Display of an f64: the shortest decimal that round-trips, never in exponent form
+// (2.0 -> "2", 0.1 -> "0.1").
+inline std::string display_f64(double x) {
+  char buf[512];
+  const auto r = std::to_chars(buf, buf + sizeof buf, x, std::chars_format::fixed);
+  return std::string(buf, r.ptr);
+}
+// Rust's Debug of an f64 as it appears in gate names: Display plus ".0" when integral.
+inline std::string debug_f64(double x) {
+  std::string s = display_f64(x);
+  if (s.find_first_of(".nN") == std::string::npos) s += ".0";
+  return s;
 }
 inline bool is_ascii(const std::string& s) {
   for (unsigned char c : s)
@@ -219,6 +233,78 @@ class SuperPosition {
     amplitudes_ = a;
     return *this;
   }
+  // super_positions.rs:105-131 (keys: product states of one dimension; missing states are 0).
+  static SuperPosition new_with_hash_amplitudes(const std::map<ProductState, Complex64>& h) {
+    if (h.empty())
+      throw QuantrError("An empty HashMap was given. A superposition must have at least one non-zero state.");
+    const size_t dim = h.begin()->first.num_qubits();
+    double total = 0.0;
+    for (const auto& kv : h) {
+      if (kv.first.num_qubits() != dim) {
+        std::ostringstream os;
+        os << "The first state has product dimension of " << dim << ", whilst the state, |" << kv.first.to_string()
+           << ">, found as a key in the HashMap has dimension " << kv.first.num_qubits() << ".";
+        throw QuantrError(os.str());
+      }
+      total += std::norm(kv.second);
+    }
+    if (!equal_within_error(total, 1.0))
+      throw QuantrError("The total sum of the absolute square of all amplitudes, " + detail::display_f64(total) +
+                        ", does not equal 1. That is, the superpositon does not conserve probability.");
+    SuperPosition s;
+    s.product_dim_ = dim;
+    s.amplitudes_.assign(size_t(1) << dim, Complex64(0, 0));
+    for (const auto& kv : h) s.amplitudes_[kv.first.comp_basis()] = kv.second;
+    return s;
+  }
+  // super_positions.rs:270-304
+  SuperPosition& set_amplitudes_from_states(const std::map<ProductState, Complex64>& h) {
+    if (h.empty())
+      throw QuantrError("An empty HashMap was given. A superposition must have at least one non-zero state.");
+    double total = 0.0;
+    for (const auto& kv : h) {
+      if (kv.first.num_qubits() != product_dim_) {
+        std::ostringstream os;
+        os << "The first state has product dimension of " << product_dim_ << ", whilst the state, |"
+           << kv.first.to_string() << ">, found as a key in the HashMap has dimension " << kv.first.num_qubits()
+           << ".";
+        throw QuantrError(os.str());
+      }
+      total += std::norm(kv.second);
+    }
+    if (!equal_within_error(total, 1.0))
+      throw QuantrError(
+          "The total sum of the absolute square of all amplitudes does not equal 1. That is, the superpositon does "
+          "not conserve probability.");
+    std::fill(amplitudes_.begin(), amplitudes_.end(), Complex64(0, 0));
+    for (const auto& kv : h) amplitudes_[kv.first.comp_basis()] = kv.second;
+    return *this;
+  }
+  // From<Qubit> (super_positions.rs:378-392)
+  static SuperPosition from(Qubit q) { return from(ProductState(q)); }
+  // super_positions.rs:332-342 on the host: one draw from the thread-local generator, left fold
+  // of |amplitude|^2, the first state whose running sum exceeds the draw (none if the total
+  // probability is below it).
+  std::optional<ProductState> measure() const;
+  // Iteration (super_position_iter.rs:56-72): every basis state in index order, zeros included.
+  class iterator {
+   public:
+    iterator(const SuperPosition* s, size_t i) : s_(s), i_(i) {}
+    std::pair<ProductState, Complex64> operator*() const {
+      return {ProductState::binary_basis(i_, s_->product_dim_), s_->amplitudes_[i_]};
+    }
+    iterator& operator++() {
+      ++i_;
+      return *this;
+    }
+    bool operator!=(const iterator& o) const { return i_ != o.i_; }
+
+   private:
+    const SuperPosition* s_;
+    size_t i_;
+  };
+  iterator begin() const { return iterator(this, 0); }
+  iterator end() const { return iterator(this, amplitudes_.size()); }
   // Non-zero amplitudes keyed by basis state (super_positions.rs:315-323).
   std::map<ProductState, Complex64> to_hash_map() const {
     std::map<ProductState, Complex64> m;
@@ -241,6 +327,16 @@ class SuperPosition {
   std::vector<Complex64> amplitudes_;
   size_t product_dim_ = 0;
 };
+
+inline std::optional<ProductState> SuperPosition::measure() const {
+  double cum = 0.0;
+  const double u = detail::rng().f64();
+  for (size_t i = 0; i < amplitudes_.size(); ++i) {
+    cum += amplitudes_[i].real() * amplitudes_[i].real() + amplitudes_[i].imag() * amplitudes_[i].imag();
+    if (u < cum) return ProductState::binary_basis(i, product_dim_);
+  }
+  return std::nullopt;
+}
 
 }  // namespace states
 
@@ -392,14 +488,7 @@ class Gate {
  private:
   explicit Gate(Kind k, double a = 0.0, int32_t kk = 0, std::vector<size_t> nodes = {})
       : kind_(k), angle_(a), k_(kk), nodes_(std::move(nodes)) {}
-  static std::string fmt(double x) {
-    std::ostringstream os;
-    os.precision(17);
-    os << x;
-    std::string s = os.str();
-    if (s.find_first_of(".eEn") == std::string::npos) s += ".0";
-    return s;
-  }
+  static std::string fmt(double x) { return detail::debug_f64(x); }
   Kind kind_ = Kind::Id;
   double angle_ = 0.0;
   int32_t k_ = 0;
@@ -841,6 +930,124 @@ inline Measurement<SimulatedCircuit::Bins> SimulatedCircuit::measure_all_without
   flush();
   return Measurement<Bins>::Observable(std::move(bins));
 }
+
+// ---- printer.rs:17-377: text diagram of a circuit ----------------------------------------------
+// One block of four text rows per wire (box top, wire with the gate name, box bottom,
+// connection row) and one block column per gate column; a column holding a multi-wire gate
+// draws its controls (█), crossings (┼) and the connecting line (│).
+class Printer {
+ public:
+  explicit Printer(const Circuit& c) : circuit_(&c) {}
+  void print_diagram() {
+    if (columns() > 14 && !disable_warnings_)
+      std::fprintf(stderr, "%s\n",
+                   "\x1b[93m[Quantr Warning] The string displaying the circuit diagram exceeds 72 chars, which could "
+                   "cause the circuit to render incorrectly in terminals (due to the wrapping). Instead, consider "
+                   "saving the string to a .txt file by using Printer::save_diagram.\x1b[0m");
+    std::printf("%s\n", get_diagram().c_str());
+  }
+  bool save_diagram(const std::string& path) {
+    std::FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return false;
+    const std::string d = get_diagram();
+    const bool ok = std::fwrite(d.data(), 1, d.size(), f) == d.size();
+    return std::fclose(f) == 0 && ok;
+  }
+  bool print_and_save_diagram(const std::string& path) {
+    std::printf("%s\n", get_diagram().c_str());
+    return save_diagram(path);
+  }
+  std::string get_diagram() {
+    if (!diagram_) diagram_ = make_diagram();
+    return *diagram_;
+  }
+  void set_warnings(bool printing) { disable_warnings_ = printing; }
+
+ private:
+  struct Rows {
+    std::string top, name, bottom, link;
+  };
+  static std::string rep(const char* s, size_t n) {
+    std::string out;
+    for (size_t i = 0; i < n; ++i) out += s;
+    return out;
+  }
+  size_t columns() const { return circuit_->get_gates().size() / circuit_->get_num_qubits(); }
+  std::string make_diagram() const {
+    const size_t n = circuit_->get_num_qubits();
+    const auto& gates = circuit_->get_gates();
+    std::vector<std::string> rows(4 * n + 1);
+    auto put = [&](size_t wire, const Rows& r) {
+      rows[4 * wire] += r.top;
+      rows[4 * wire + 1] += r.name;
+      rows[4 * wire + 2] += r.bottom;
+      rows[4 * wire + 3] += r.link;
+    };
+    for (size_t col = 0; col < columns(); ++col) {
+      const Gate* g = &gates[col * n];
+      size_t widest = 1, multi = n;
+      for (size_t w = 0; w < n; ++w) {
+        widest = std::max(widest, g[w].get_name().size());
+        if (multi == n && !g[w].is_single_gate()) multi = w;
+      }
+      if (multi < n) {
+        draw_multi(put, g[multi], multi, n);
+        continue;
+      }
+      for (size_t w = 0; w < n; ++w) {
+        const std::string name = g[w].get_name();
+        const size_t pad = widest - name.size();
+        Rows r;
+        if (g[w].kind() == Gate::Kind::Id) {
+          r = {rep(" ", widest + 4), rep("\u2500", widest + 4), rep(" ", widest + 4), rep(" ", widest + 4)};
+        } else {
+          r.top = "\u250f\u2501" + rep("\u2501", name.size()) + "\u2501\u2513" + rep(" ", pad);
+          r.name = "\u2528 " + name + " \u2520" + rep("\u2500", pad);
+          r.bottom = "\u2517\u2501" + rep("\u2501", name.size()) + "\u2501\u251b" + rep(" ", pad);
+          r.link = rep(" ", widest + 4);
+        }
+        put(w, r);
+      }
+    }
+    std::string out;
+    for (const auto& r : rows) out += r + "\n";
+    return out;
+  }
+  template <class Put>
+  static void draw_multi(Put& put, const Gate& g, size_t pos, size_t n) {
+    std::vector<size_t> nodes = *g.get_nodes();
+    nodes.push_back(pos);
+    const size_t lo = *std::min_element(nodes.begin(), nodes.end());
+    const size_t hi = *std::max_element(nodes.begin(), nodes.end());
+    const std::string name = g.get_name();
+    const size_t L = name.size();
+    const std::string bar = "  \u2502  " + rep(" ", L - 1);
+    for (size_t w = 0; w < n; ++w) {
+      Rows r;
+      if (w == pos) {
+        r.top = std::string("\u250f\u2501") + (pos > lo ? "\u2537" : "\u2501") + rep("\u2501", L - 1) + "\u2501\u2513";
+        r.name = "\u2528 " + name + " \u2520";
+        r.bottom = std::string("\u2517\u2501") + (pos < hi ? "\u252f" : "\u2501") + rep("\u2501", L - 1) + "\u2501\u251b";
+        r.link = std::string("  ") + (pos < hi ? "\u2502" : " ") + rep(" ", L + 1);
+      } else if (w == lo) {
+        r = {rep(" ", L + 4), "\u2500\u2500\u2588\u2500\u2500" + rep("\u2500", L - 1), bar, bar};
+      } else if (w == hi) {
+        r = {bar, "\u2500\u2500\u2588\u2500\u2500" + rep("\u2500", L - 1), rep(" ", L + 4), rep(" ", L + 4)};
+      } else if (std::find(nodes.begin(), nodes.end(), w) != nodes.end()) {
+        r = {bar, "\u2500\u2500\u2588\u2500\u2500" + rep("\u2500", L - 1), bar, bar};
+      } else if (w > lo && w < hi) {
+        r = {bar, "\u2500\u2500\u253c\u2500\u2500" + rep("\u2500", L - 1), bar, bar};
+      } else {
+        r = {rep(" ", L + 4), rep("\u2500", 5) + rep("\u2500", L - 1), " " + rep(" ", L + 3), " " + rep(" ", L + 3)};
+      }
+      put(w, r);
+    }
+  }
+
+  const Circuit* circuit_;
+  std::optional<std::string> diagram_;
+  bool disable_warnings_ = false;
+};
 
 // ---- adjoint differentiation (no reference counterpart) -------------------------------------
 // forward(): E = <psi|O|psi> for psi = U(theta)|0...0>; backward(g): g * dE/dtheta for every

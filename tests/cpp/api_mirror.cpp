@@ -3,6 +3,7 @@
 //
 //   api_mirror cpu   builder, layout, error and host-state tests (no device calls)
 //   api_mirror gpu   simulations through the engine on the GPU
+//   api_mirror diagrams   the Printer's diagrams of the reference's printer tests
 //
 // Each test prints "ok <name>" or "FAIL <name>: <why>"; the exit status is the failure count.
 #include <cmath>
@@ -223,6 +224,42 @@ void cpu_tests() {
                  "position must be strictly less.");
     check(p.kronecker_prod(Qubit::Zero).to_string() == "10110", "", "kron");
   });
+  run("super_position_hash_api", [] {  // super_positions.rs:92-131, 255-304 (doc examples)
+    const ProductState p01 = ProductState::new_({Qubit::Zero, Qubit::One});
+    SuperPosition a = SuperPosition::new_with_hash_amplitudes({{p01, Complex64(1, 0)}});
+    check(a.get_amplitudes() == std::vector<Complex64>{0, 1, 0, 0}, "", "new_with_hash_amplitudes");
+    SuperPosition b = SuperPosition::new_(2);
+    b.set_amplitudes_from_states({{p01, Complex64(1, 0)}});
+    check(b.get_amplitudes() == std::vector<Complex64>{0, 1, 0, 0}, "", "set_amplitudes_from_states");
+    expect_error([] { SuperPosition::new_with_hash_amplitudes({}); },
+                 "An empty HashMap was given. A superposition must have at least one non-zero state.");
+    expect_error([&] { SuperPosition::new_with_hash_amplitudes({{p01, Complex64(1, 1)}}); },
+                 "The total sum of the absolute square of all amplitudes, 2, does not equal 1. That is, the "
+                 "superpositon does not conserve probability.");
+    expect_error([&] {
+      SuperPosition::new_with_hash_amplitudes({{p01, Complex64(kS, 0)}, {ProductState::new_({Qubit::One}), Complex64(kS, 0)}});
+    }, ">, found as a key in the HashMap has dimension ", true);  // which key is "first" follows map order
+    expect_error([&] { b.set_amplitudes_from_states({{ProductState::new_({Qubit::One}), Complex64(1, 0)}}); },
+                 "The first state has product dimension of 2, whilst the state, |1>, found as a key in the HashMap has "
+                 "dimension 1.");
+  });
+  run("super_position_iteration_and_measure", [] {  // super_position_iter.rs:56-72, super_positions.rs:332-342
+    SuperPosition s = SuperPosition::new_(2);
+    s.set_amplitudes({Complex64(0, 0), Complex64(0, 0), Complex64(0, 1), Complex64(0, 0)});
+    std::vector<std::string> seen;
+    for (const auto& [state, amp] : s) seen.push_back(state.to_string() + (std::abs(amp) > 0 ? "*" : ""));
+    check(seen == std::vector<std::string>{"00", "01", "10*", "11"}, "", "iteration order");
+    quantr::seed(3);
+    for (int i = 0; i < 20; ++i) check(s.measure()->to_string() == "10", "", "measure of a basis state");
+    SuperPosition z = SuperPosition::new_with_amplitudes_unchecked({Complex64(0, 0), Complex64(0, 0)});
+    check(!z.measure().has_value(), "", "measure with total probability 0");
+    check(SuperPosition::from(Qubit::One).get_amplitudes() == std::vector<Complex64>{0, 1}, "", "from qubit");
+  });
+  run("gate_debug_floats", [] {  // Rust Debug of f64 in messages
+    check(Gate::Rx(0.1).debug() == "Rx(0.1)" && Gate::Phase(2.0).debug() == "Phase(2.0)" &&
+              Gate::CR(-1.5, 0).debug() == "CR(-1.5, 0)",
+          "", Gate::Rx(0.1).debug());
+  });
   run("super_position_api", [] {  // super_positions.rs
     expect_error([] { SuperPosition::new_with_amplitudes({Complex64(1, 0), Complex64(1, 0)}); },
                  "Slice given to set amplitudes in super position does not conserve probability, the absolute "
@@ -377,10 +414,39 @@ void gpu_tests() {
 
 }  // namespace
 
+// Diagrams of the reference's Printer tests (printer.rs:395-440), printed between markers for
+// comparison with the fixture.
+void diagrams() {
+  {
+    Circuit qc = Circuit::new_(4);
+    qc.add_gate(Gate::H(), 3)
+        .add_repeating_gate(Gate::Y(), {0, 1})
+        .add_gate(Gate::Toffoli(0, 3), 1)
+        .add_gate(Gate::CNot(1), 3)
+        .add_gate(Gate::CNot(2), 0)
+        .add_gate(Gate::CNot(2), 1);
+    quantr::Printer p(qc);
+    std::printf("<<<producing_string_circuit\n%s>>>\n", p.get_diagram().c_str());
+  }
+  {
+    Circuit qc = Circuit::new_(4);
+    qc.add_gate(Gate::H(), 3);
+    qc.add_gates({Gate::H(), Gate::Custom(example_cnot, {3}, "Custom CNot"), Gate::Id(), Gate::X()})
+        .add_repeating_gate(Gate::Y(), {0, 1})
+        .add_gate(Gate::Toffoli(0, 3), 1)
+        .add_gate(Gate::CNot(1), 3)
+        .add_gate(Gate::CNot(2), 0)
+        .add_gate(Gate::CNot(2), 1);
+    quantr::Printer p(qc);
+    std::printf("<<<producing_string_circuit_custom\n%s>>>\n", p.get_diagram().c_str());
+  }
+}
+
 int main(int argc, char** argv) {
   const std::string mode = argc > 1 ? argv[1] : "cpu";
   if (mode == "cpu") cpu_tests();
   else if (mode == "gpu") gpu_tests();
+  else if (mode == "diagrams") diagrams();
   else {
     std::fprintf(stderr, "usage: api_mirror cpu|gpu\n");
     return 2;

@@ -41,6 +41,18 @@ def test_mirror_builder_and_errors():
     assert out.count("ok ") >= 16
 
 
+def test_mirror_printer_diagrams():
+    """Printer output equals the diagrams asserted by the reference's printer tests."""
+    import json
+    exe = build()
+    out = subprocess.run([exe, "diagrams"], capture_output=True, timeout=60).stdout.decode("utf-8")
+    fixture = json.load(open(os.path.join(H.ROOT, "tests", "golden", "printer_diagrams.json"), encoding="utf-8"))
+    for case in fixture["cases"]:
+        start = out.index("<<<" + case["name"] + "\n") + len(case["name"]) + 4
+        got = out[start:out.index(">>>", start)]
+        assert got == case["diagram"], (case["name"], got)
+
+
 @pytest.mark.gpu
 def test_mirror_simulations_gpu():
     out = run_mode("gpu", 240)
