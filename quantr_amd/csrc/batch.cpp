@@ -34,12 +34,6 @@ constexpr size_t kAlign = 256;
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-struct Group {
-  uint64_t xm;
-  size_t off;  // byte offset of the group's PauliZTerm array in the blob
-  int count;
-};
-
 struct Inst {
   uint32_t n = 0;
   uint64_t dim = 0;
@@ -51,7 +45,8 @@ struct Inst {
   std::vector<Loc> locf, locb;
   size_t blob_f = 0, blob_b = 0;  // offsets of the two plan blobs in the batch blob
   size_t blob_f_size = 0, blob_b_size = 0;
-  std::vector<Group> groups;
+  size_t obs_groups = 0, obs_terms = 0;  // byte offsets of the observable's tables in the blob
+  int ngroups = 0;
   uint64_t nparams = 0, nslots = 0;
   std::vector<int> gens;      // generator of each parameter (4 = global phase)
   size_t psi = 0, lam = 0;    // byte offsets in the state pool
@@ -185,15 +180,10 @@ void launch_instance(qr_batch* b, const Inst& I, hipStream_t s, uint64_t* stamps
       launch_standalone_raw(prec, I.dim, psi, b->tmp[I.stream], I.fwd[st.standalone], dblob + I.blob_f, I.locf[si], s);
     }
   }
-  // lam = O psi, E = Re <psi|lam>
-  if (I.groups.empty()) {
-    hip_check(launch_pauli_group(prec, lam, psi, I.dim, 0, nullptr, 0, 0, s), "pauli group");
-  } else {
-    for (size_t g = 0; g < I.groups.size(); ++g)
-      hip_check(launch_pauli_group(prec, lam, psi, I.dim, I.groups[g].xm, (const PauliZTerm*)(dblob + I.groups[g].off),
-                                   I.groups[g].count, g ? 1 : 0, s),
-                "pauli group");
-  }
+  // lam = O psi (one pass), E = Re <psi|lam>
+  hip_check(launch_pauli_sum(prec, lam, psi, I.dim, (const PauliGroup*)(dblob + I.obs_groups), I.ngroups,
+                             (const PauliZTerm*)(dblob + I.obs_terms), s),
+            "pauli sum");
   hip_check(launch_dot_re_partial(prec, psi, lam, I.dim, work + I.dot, kDotBlocks, s), "dot");
   hip_check(launch_sum_rows(work + I.dot, kDotBlocks, 1, outd + idx, s), "expectation sum");
   // reverse sweep
@@ -310,22 +300,16 @@ qr_batch* batch_create_widths(uint64_t m, const uint32_t* nq, const qr_program* 
     I.blob_b = blob_size = align_up(blob_size, kAlign);
     I.blob_b_size = pack_blob(I.rev, I.pb->stages, I.locb).size();
     blob_size += I.blob_b_size;
-    // Pauli groups: terms sharing an X mask, i^ny folded into the coefficient
-    std::map<uint64_t, std::vector<PauliZTerm>> groups;
-    for (const auto& tm : I.terms) {
-      double cr = tm.coeff, ci = 0.0;
-      for (int q = 0; q < (tm.ny & 3); ++q) {
-        const double t = cr;
-        cr = -ci;
-        ci = t;
-      }
-      groups[tm.xm].push_back(PauliZTerm{tm.zm, cr, ci});
-    }
-    I.groups.clear();
-    for (const auto& g : groups) {
-      blob_size = align_up(blob_size, 16);
-      I.groups.push_back(Group{g.first, blob_size, (int)g.second.size()});
-      blob_size += g.second.size() * sizeof(PauliZTerm);
+    // the observable's tables: terms grouped by X mask
+    {
+      std::vector<PauliGroup> groups;
+      std::vector<PauliZTerm> flat;
+      pauli_tables(I.terms, groups, flat);
+      I.ngroups = (int)groups.size();
+      I.obs_groups = blob_size = align_up(blob_size, 16);
+      blob_size += groups.size() * sizeof(PauliGroup);
+      I.obs_terms = blob_size = align_up(blob_size, 16);
+      blob_size += flat.size() * sizeof(PauliZTerm);
     }
     I.psi = state_bytes = align_up(state_bytes, kAlign);
     state_bytes += I.dim * B;
@@ -339,22 +323,13 @@ qr_batch* batch_create_widths(uint64_t m, const uint32_t* nq, const qr_program* 
     b->total_params += I.nparams;
   }
   b->host_blob.assign(blob_size, 0);
-  for (Inst& I : b->inst) {  // Pauli tables (fixed for the batch's lifetime)
-    std::map<uint64_t, std::vector<PauliZTerm>> groups;
-    for (const auto& tm : I.terms) {
-      double cr = tm.coeff, ci = 0.0;
-      for (int q = 0; q < (tm.ny & 3); ++q) {
-        const double t = cr;
-        cr = -ci;
-        ci = t;
-      }
-      groups[tm.xm].push_back(PauliZTerm{tm.zm, cr, ci});
-    }
-    size_t g = 0;
-    for (const auto& kv : groups) {
-      std::memcpy(b->host_blob.data() + I.groups[g].off, kv.second.data(), kv.second.size() * sizeof(PauliZTerm));
-      ++g;
-    }
+  for (Inst& I : b->inst) {  // observable tables (fixed for the batch's lifetime)
+    std::vector<PauliGroup> groups;
+    std::vector<PauliZTerm> flat;
+    pauli_tables(I.terms, groups, flat);
+    if (!groups.empty())
+      std::memcpy(b->host_blob.data() + I.obs_groups, groups.data(), groups.size() * sizeof(PauliGroup));
+    if (!flat.empty()) std::memcpy(b->host_blob.data() + I.obs_terms, flat.data(), flat.size() * sizeof(PauliZTerm));
   }
   fill_blob(b.get());
   const size_t out_doubles = m + b->total_params;

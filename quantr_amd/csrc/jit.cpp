@@ -90,7 +90,8 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   };
   std::ostringstream o;
   o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")"
-    << (persistent ? " __attribute__((amdgpu_waves_per_eu(2, 2)))" : "") << " void qr_stage(" << C
+    << (persistent ? " __attribute__((amdgpu_waves_per_eu(2, 2)))" : "") << " void "
+    << (dual ? "qr_dual_stage(" : "qr_stage(") << C
     << "* __restrict__ psi, ";
   if (dual) o << C << "* __restrict__ lam, ";
   o << "const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles";
@@ -335,10 +336,18 @@ std::string compiler_path() {
   return access(p.c_str(), X_OK) == 0 ? p : std::string();
 }
 
-std::string renamed(const std::string& kernel, const std::string& name) {
+// Entry point of a stage source: qr_stage (forward) or qr_dual_stage (adjoint sweep).
+std::string entry_name(const std::string& kernel) {
+  return kernel.find("void qr_dual_stage(") != std::string::npos ? "qr_dual_stage" : "qr_stage";
+}
+
+// The source with its entry point renamed to <entry>_<j> (unique within a group; the prefix
+// keeps profiles grouped by kind).
+std::string renamed(const std::string& kernel, size_t j) {
   std::string k = kernel;
-  const size_t at = k.find("void qr_stage(");
-  if (at != std::string::npos) k.replace(at, 14, "void " + name + "(");
+  const std::string e = entry_name(kernel);
+  const size_t at = k.find("void " + e + "(");
+  if (at != std::string::npos) k.replace(at, 6 + e.size(), "void " + e + "_" + std::to_string(j) + "(");
   return k;
 }
 
@@ -376,7 +385,7 @@ void compile_in_children(const std::vector<std::string>& todo, std::vector<Group
     G.log = d + "/g" + std::to_string(g) + ".log";
     std::ofstream f(G.src);
     f << "#include <hip/hip_runtime.h>\n" << kTileOpsSource << "\n";
-    for (size_t j = 0; j < G.items.size(); ++j) f << renamed(todo[G.items[j]], "qr_stage_" + std::to_string(j));
+    for (size_t j = 0; j < G.items.size(); ++j) f << renamed(todo[G.items[j]], j);
     f.close();
     if (!f) continue;
     std::vector<std::string> args = {cc, "-x", "hip", "--offload-arch=gfx950", "--cuda-device-only",
@@ -433,7 +442,8 @@ void jit_prepare(const std::vector<std::string>& sources) {
         if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
           for (size_t j = 0; j < G.items.size(); ++j) {
             hipFunction_t fn = nullptr;
-            if (hipModuleGetFunction(&fn, mod, ("qr_stage_" + std::to_string(j)).c_str()) == hipSuccess) {
+            const std::string nm = entry_name(todo[G.items[j]]) + "_" + std::to_string(j);
+            if (hipModuleGetFunction(&fn, mod, nm.c_str()) == hipSuccess) {
               made[G.items[j]] = JitKernel{mod, fn};
               done[G.items[j]] = 1;
             }
@@ -457,7 +467,7 @@ void jit_prepare(const std::vector<std::string>& sources) {
     JitKernel k;
     hipError_t e = hipModuleLoadData(&k.mod, code.data());
     if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e)};
-    e = hipModuleGetFunction(&k.fn, k.mod, "qr_stage");
+    e = hipModuleGetFunction(&k.fn, k.mod, entry_name(todo[i]).c_str());
     if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleGetFunction: ") + hipGetErrorString(e)};
     made[i] = k;
     done[i] = 1;

@@ -649,27 +649,6 @@ __global__ __launch_bounds__(256) void k_zdiag_partial(const typename CT<R>::T* 
 
 // lam (+)= c * i^ny * (-1)^popcount(j&z) psi_{j^x}   evaluated at output index j' = j ^ x:
 // (P psi)_o = i^ny (-1)^popcount((o ^ x) & z) psi_{o ^ x}.
-template <typename R>
-__global__ void k_pauli_apply(typename CT<R>::T* __restrict__ lam, const typename CT<R>::T* __restrict__ psi,
-                              uint64_t dim, uint64_t xm, uint64_t zm, double cr, double ci, int accumulate) {
-  using C = typename CT<R>::T;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < dim; o += stride) {
-    const uint64_t j = o ^ xm;
-    const double sg = (__popcll(j & zm) & 1) ? -1.0 : 1.0;
-    const double vr = (double)psi[j].x * sg, vi = (double)psi[j].y * sg;
-    double yr = cr * vr - ci * vi, yi = cr * vi + ci * vr;
-    if (accumulate) {
-      yr += (double)lam[o].x;
-      yi += (double)lam[o].y;
-    }
-    C w;
-    w.x = (R)yr;
-    w.y = (R)yi;
-    lam[o] = w;
-  }
-}
-
 // Gradient term of one rotation gate: sum_j conj(lam_j) (G psi)_j for G in {X, Y, Z, I} on bit t.
 // Returns partial (re, im). Y: (Y psi)_j = -i (-1)^{bit_t(j)} psi_{j^t}; Z: (-1)^{bit_t(j)} psi_j.
 template <typename R>
@@ -715,32 +694,29 @@ __global__ __launch_bounds__(256) void k_gen_partial(const typename CT<R>::T* __
   if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
 }
 
-// lam[o] (+)= sum_t c_t (-1)^{popcount(j & zm_t)} psi[j], j = o ^ xm: every Pauli term of one
-// X-mask group in one pass (terms read with uniform loads).
+// lam = O psi for a whole Pauli sum in one pass: each output adds one product per X-mask
+// group (groups and terms read with uniform loads).
 template <typename R>
-__global__ __launch_bounds__(256) void k_pauli_group(typename CT<R>::T* __restrict__ lam,
-                                                     const typename CT<R>::T* __restrict__ psi, uint64_t dim,
-                                                     uint64_t xm, const PauliZTerm* __restrict__ terms, int nterms,
-                                                     int accumulate) {
+__global__ __launch_bounds__(256) void k_pauli_sum(typename CT<R>::T* __restrict__ lam,
+                                                   const typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                   const PauliGroup* __restrict__ groups, int ngroups,
+                                                   const PauliZTerm* __restrict__ terms) {
   using C = typename CT<R>::T;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < dim; o += stride) {
-    const uint64_t j = o ^ xm;
-    double fr = 0.0, fi = 0.0;
-    for (int t = 0; t < nterms; ++t) {
-      const bool neg = __popcll(j & terms[t].zm) & 1;
-      fr += neg ? -terms[t].cr : terms[t].cr;
-      fi += neg ? -terms[t].ci : terms[t].ci;
-    }
     double yr = 0.0, yi = 0.0;
-    if (nterms) {
+    for (int g = 0; g < ngroups; ++g) {
+      const uint64_t j = o ^ groups[g].xm;
+      double fr = 0.0, fi = 0.0;
+      const uint32_t t0 = groups[g].first, t1 = t0 + groups[g].count;
+      for (uint32_t t = t0; t < t1; ++t) {
+        const bool neg = __popcll(j & terms[t].zm) & 1;
+        fr += neg ? -terms[t].cr : terms[t].cr;
+        fi += neg ? -terms[t].ci : terms[t].ci;
+      }
       const double pr = (double)psi[j].x, pi = (double)psi[j].y;
-      yr = fr * pr - fi * pi;
-      yi = fr * pi + fi * pr;
-    }
-    if (accumulate) {
-      yr += (double)lam[o].x;
-      yi += (double)lam[o].y;
+      yr += fr * pr - fi * pi;
+      yi += fr * pi + fi * pr;
     }
     C w;
     w.x = (R)yr;
@@ -1038,16 +1014,6 @@ hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const u
   return hipGetLastError();
 }
 
-hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, uint64_t zm,
-                              double cr, double ci, int accumulate, hipStream_t s) {
-  const unsigned g = grid_for(dim, 256, 1u << 18);
-  if (prec == 0)
-    dev::k_pauli_apply<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, xm, zm, cr, ci, accumulate);
-  else
-    dev::k_pauli_apply<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, xm, zm, cr, ci, accumulate);
-  return hipGetLastError();
-}
-
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s) {
   if (prec == 0)
@@ -1057,13 +1023,13 @@ hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64
   return hipGetLastError();
 }
 
-hipError_t launch_pauli_group(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, const PauliZTerm* terms,
-                              int nterms, int accumulate, hipStream_t s) {
+hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
+                            const PauliZTerm* terms, hipStream_t s) {
   const unsigned g = grid_for(dim, 256, 1u << 18);
   if (prec == 0)
-    dev::k_pauli_group<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, xm, terms, nterms, accumulate);
+    dev::k_pauli_sum<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, groups, ngroups, terms);
   else
-    dev::k_pauli_group<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, xm, terms, nterms, accumulate);
+    dev::k_pauli_sum<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, groups, ngroups, terms);
   return hipGetLastError();
 }
 

@@ -105,8 +105,6 @@ hipError_t launch_pauli_partial(int prec, const void* phi, const void* psi, uint
 hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const uint64_t* zmasks,
                                 const double* coeffs, int nterms, double* part, int nblocks,
                                 hipStream_t s);
-hipError_t launch_pauli_apply(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, uint64_t zm,
-                              double cr, double ci, int accumulate, hipStream_t s);
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s);
 // One Pauli-sum term of an X-mask group: coefficient (with i^ny folded in) times
@@ -115,9 +113,15 @@ struct PauliZTerm {
   uint64_t zm;
   double cr, ci;
 };
-// lam[o] (+)= sum_t c_t (-1)^{popcount(j & zm_t)} psi[j], j = o ^ xm; nterms = 0 writes zeros.
-hipError_t launch_pauli_group(int prec, void* lam, const void* psi, uint64_t dim, uint64_t xm, const PauliZTerm* terms,
-                              int nterms, int accumulate, hipStream_t s);
+// An X-mask group of a Pauli sum: terms[first .. first + count) share X mask xm.
+struct PauliGroup {
+  uint64_t xm;
+  uint32_t first, count;
+};
+// lam[o] = sum over groups g, terms t of g: c_t (-1)^{popcount(j & zm_t)} psi[j], j = o ^ xm_g
+// (the whole observable in one pass).
+hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
+                            const PauliZTerm* terms, hipStream_t s);
 // part[b] = partial sums of Re sum_j conj(a_j) b_j over nblocks blocks.
 hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
                                  hipStream_t s);

@@ -528,11 +528,10 @@ double expectation(qr_state* st, const std::vector<PauliMasks>& terms) {
 }
 
 
-void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const std::vector<PauliMasks>& terms,
-                      DevBuf& scratch, hipStream_t s) {
-  // group the terms by X mask: one pass per group, Z masks and i^ny folded into per-term
-  // complex coefficients
-  std::map<uint64_t, std::vector<PauliZTerm>> groups;
+void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
+                  std::vector<PauliZTerm>& flat) {
+  // terms grouped by X mask; Z masks and i^ny folded into per-term complex coefficients
+  std::map<uint64_t, std::vector<PauliZTerm>> by_x;
   for (const auto& tm : terms) {
     double cr = tm.coeff, ci = 0.0;
     for (int q = 0; q < (tm.ny & 3); ++q) {
@@ -540,26 +539,32 @@ void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const 
       cr = -ci;
       ci = t;
     }
-    groups[tm.xm].push_back(PauliZTerm{tm.zm, cr, ci});
+    by_x[tm.xm].push_back(PauliZTerm{tm.zm, cr, ci});
   }
-  if (groups.empty()) {
-    hip_check(launch_pauli_group(prec, lam, psi, dim, 0, nullptr, 0, 0, s), "pauli group");
-    return;
+  groups.clear();
+  flat.clear();
+  for (const auto& g : by_x) {
+    groups.push_back(PauliGroup{g.first, (uint32_t)flat.size(), (uint32_t)g.second.size()});
+    flat.insert(flat.end(), g.second.begin(), g.second.end());
   }
+}
+
+void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const std::vector<PauliMasks>& terms,
+                      DevBuf& scratch, hipStream_t s) {
+  std::vector<PauliGroup> groups;
   std::vector<PauliZTerm> flat;
-  for (const auto& g : groups) flat.insert(flat.end(), g.second.begin(), g.second.end());
-  scratch.reserve(flat.size() * sizeof(PauliZTerm));
-  hip_check(hipMemcpyAsync(scratch.p, flat.data(), flat.size() * sizeof(PauliZTerm), hipMemcpyHostToDevice, s), "H2D");
-  size_t at = 0;
-  bool first = true;
-  for (const auto& g : groups) {
-    hip_check(launch_pauli_group(prec, lam, psi, dim, g.first, (const PauliZTerm*)scratch.p + at, (int)g.second.size(),
-                                 first ? 0 : 1, s),
-              "pauli group");
-    at += g.second.size();
-    first = false;
-  }
-  // the term table must outlive the launches that read it
+  pauli_tables(terms, groups, flat);
+  const size_t gb = groups.size() * sizeof(PauliGroup), tb = flat.size() * sizeof(PauliZTerm);
+  scratch.reserve(gb + tb + 16);
+  std::vector<uint8_t> blob(gb + tb);
+  if (gb) std::memcpy(blob.data(), groups.data(), gb);
+  if (tb) std::memcpy(blob.data() + gb, flat.data(), tb);
+  if (!blob.empty())
+    hip_check(hipMemcpyAsync(scratch.p, blob.data(), blob.size(), hipMemcpyHostToDevice, s), "H2D");
+  hip_check(launch_pauli_sum(prec, lam, psi, dim, (const PauliGroup*)scratch.p, (int)groups.size(),
+                             (const PauliZTerm*)((uint8_t*)scratch.p + gb), s),
+            "pauli sum");
+  // the tables must outlive the launch that reads them
   hip_check(hipStreamSynchronize(s), "sync");
 }
 

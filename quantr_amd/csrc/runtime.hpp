@@ -168,7 +168,10 @@ double sum_partials_host(const std::vector<double>& v);
 double norm_sqr(qr_state* st);
 std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o);
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms);
-// lam = O psi for an observable, on stream s (terms grouped by X mask, one launch per group).
+// The observable as device tables: terms grouped by X mask (i^ny folded into coefficients).
+void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
+                  std::vector<PauliZTerm>& flat);
+// lam = O psi for an observable, on stream s (one pass over the state).
 void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const std::vector<PauliMasks>& terms,
                       DevBuf& scratch, hipStream_t s);
 void backward_fused(qr_grad* g, double grad_out, double* grads_out);

@@ -15,14 +15,37 @@ import json
 import sys
 
 
+def kernel_kind(name):
+    """Compiled stages are named <kind>_<j> (one entry point per group of a module); they are
+    grouped by kind."""
+    for kind in ("qr_dual_stage", "qr_stage"):
+        if name == kind or name.startswith(kind + "_"):
+            return kind
+    return name
+
+
 def per_kernel(path, counter=None):
     d = collections.defaultdict(list)
     with open(path) as f:
         for r in csv.DictReader(f):
             if counter is not None and r["Counter_Name"] != counter:
                 continue
-            d[r["Kernel_Name"]].append(float(r["Counter_Value"]))
+            d[kernel_kind(r["Kernel_Name"])].append(float(r["Counter_Value"]))
     return {k: sum(v) / len(v) for k, v in d.items()}, {k: len(v) for k, v in d.items()}
+
+
+def stats_from_trace(trace_csv):
+    """Per-kind call count, average/min/max duration (ns) and share, from a kernel trace."""
+    d = collections.defaultdict(list)
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            d[kernel_kind(r["Kernel_Name"])].append(int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    total = sum(sum(v) for v in d.values()) or 1
+    rows = []
+    for k, v in sorted(d.items(), key=lambda kv: -sum(kv[1])):
+        rows.append({"Name": k, "Calls": len(v), "AverageNs": sum(v) / len(v), "MinNs": min(v), "MaxNs": max(v),
+                     "Percentage": "%.2f" % (100.0 * sum(v) / total)})
+    return rows
 
 
 def find(d, sub):
@@ -33,7 +56,11 @@ def find(d, sub):
 
 
 def stats(csv_path, out, cmd):
-    rows = list(csv.DictReader(open(csv_path)))
+    """csv_path: a kernel-trace CSV (grouped here by kind) or a --stats CSV."""
+    if csv_path.endswith("kernel_trace.csv"):
+        rows = stats_from_trace(csv_path)
+    else:
+        rows = list(csv.DictReader(open(csv_path)))
     lines = ["# rocprofv3 --kernel-trace --stats", "", "Command (from the repository root on the GPU host):", "",
              "    " + cmd, "", "| kernel | calls | avg ms | min ms | max ms | share % |", "|---|---|---|---|---|---|"]
     for r in rows:
