@@ -53,6 +53,7 @@ def parse():
     ap.add_argument("--no-precision", action="store_true", help="skip the complex64 drift report")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-instances", type=int, default=512)
+    ap.add_argument("--batch-streams", type=int, default=0, help="streams inside the batch graph (0 = 8)")
     ap.add_argument("--capacity", action="store_true",
                     help="also run the 33-qubit complex128 and 34-qubit complex64 capacity circuits")
     return ap.parse_args()
@@ -276,7 +277,7 @@ def bench_batch(args):
         warm = _capi.Batch(list(reps.values()), _capi.QR_C64, autotune=True)
         warm.close()
         log("bench: batch stages compiled for %d qubits (%d layer counts), %.1f s" % (n, len(reps), time.perf_counter() - t0))
-    b = _capi.Batch(items, _capi.QR_C64, autotune=True)
+    b = _capi.Batch(items, _capi.QR_C64, autotune=True, streams=args.batch_streams)
     setup = time.perf_counter() - t0
     b.run()  # warm-up
     reps = 3

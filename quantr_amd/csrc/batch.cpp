@@ -4,7 +4,8 @@
 // Creation plans every instance (forward tile stages, then the dual-state reverse sweep),
 // compiles the stages of every distinct structure, lays all states and plan blobs out in
 // three device allocations, and captures the whole batch: instances are spread over a few
-// streams (largest first, each to the least loaded stream), every launch recorded once. A run
+// streams (largest first, each to the least loaded stream; each stream then runs its own
+// instances smallest first), every launch recorded once. A run
 // uploads the programs' coefficients into the plan blob, launches the graph and reads back
 // expectation values, gradients and one device timestamp per instance.
 #include <hip/hip_runtime.h>
@@ -354,6 +355,10 @@ qr_batch* batch_create_widths(uint64_t m, const uint32_t* nq, const qr_program* 
     load[s] += b->inst[i].work;
     per_stream[s].push_back(i);
   }
+  // inside a stream, smallest first: the stream's total is unchanged, and most instances
+  // complete early instead of waiting behind the large ones
+  for (auto& list : per_stream)
+    std::stable_sort(list.begin(), list.end(), [&](size_t x, size_t y) { return b->inst[x].work < b->inst[y].work; });
   for (int s = 0; s < S; ++s) {
     size_t need = 0;
     for (size_t i : per_stream[s]) need = std::max(need, b->inst[i].tmp_bytes);
