@@ -591,64 +591,6 @@ __global__ __launch_bounds__(256) void k_norm_partial(const typename CT<R>::T* _
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
-// sum_j conj(phi_{j ^ x}) * s_z(j) * psi_j  (real and imaginary parts), s_z = (-1)^popcount(j & z).
-// With phi == psi this is <psi|P|psi> / i^ny.
-template <typename R>
-__global__ __launch_bounds__(256) void k_pauli_partial(const typename CT<R>::T* __restrict__ phi,
-                                                       const typename CT<R>::T* __restrict__ psi,
-                                                       uint64_t dim, uint64_t xm, uint64_t zm,
-                                                       double2* __restrict__ part) {
-  __shared__ double rr[256], ri[256];
-  double sr = 0.0, si = 0.0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride) {
-    const uint64_t i = j ^ xm;
-    const double ar = (double)phi[i].x, ai = -(double)phi[i].y;
-    const double br = (double)psi[j].x, bi = (double)psi[j].y;
-    const double sg = (__popcll(j & zm) & 1) ? -1.0 : 1.0;
-    sr += sg * (ar * br - ai * bi);
-    si += sg * (ar * bi + ai * br);
-  }
-  rr[threadIdx.x] = sr;
-  ri[threadIdx.x] = si;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) {
-      rr[threadIdx.x] += rr[threadIdx.x + w];
-      ri[threadIdx.x] += ri[threadIdx.x + w];
-    }
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = make_double2(rr[0], ri[0]);
-}
-
-// Diagonal Pauli terms together: sum_j |psi_j|^2 * sum_t c_t (-1)^popcount(j & z_t).
-template <typename R>
-__global__ __launch_bounds__(256) void k_zdiag_partial(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
-                                                       const uint64_t* __restrict__ zmasks,
-                                                       const double* __restrict__ coeffs, int nterms,
-                                                       double* __restrict__ part) {
-  __shared__ double red[256];
-  double s = 0.0;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t j = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; j < dim; j += stride) {
-    const double re = (double)psi[j].x, im = (double)psi[j].y;
-    const double p = re * re + im * im;
-    double w = 0.0;
-    for (int t = 0; t < nterms; ++t) w += (__popcll(j & zmasks[t]) & 1) ? -coeffs[t] : coeffs[t];
-    s += p * w;
-  }
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int w = 128; w > 0; w >>= 1) {
-    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
-}
-
-// lam (+)= c * i^ny * (-1)^popcount(j&z) psi_{j^x}   evaluated at output index j' = j ^ x:
-// (P psi)_o = i^ny (-1)^popcount((o ^ x) & z) psi_{o ^ x}.
 // Gradient term of one rotation gate: sum_j conj(lam_j) (G psi)_j for G in {X, Y, Z, I} on bit t.
 // Returns partial (re, im). Y: (Y psi)_j = -i (-1)^{bit_t(j)} psi_{j^t}; Z: (-1)^{bit_t(j)} psi_j.
 template <typename R>
@@ -723,6 +665,40 @@ __global__ __launch_bounds__(256) void k_pauli_sum(typename CT<R>::T* __restrict
     w.y = (R)yi;
     lam[o] = w;
   }
+}
+
+// Partial sums of Re <psi|O|psi> = Re sum_o conj(psi_o) (O psi)_o in one pass, one per workgroup.
+template <typename R>
+__global__ __launch_bounds__(256) void k_pauli_expect(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                      const PauliGroup* __restrict__ groups, int ngroups,
+                                                      const PauliZTerm* __restrict__ terms, double* __restrict__ part) {
+  __shared__ double red[256];
+  double acc = 0.0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t o = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; o < dim; o += stride) {
+    double yr = 0.0, yi = 0.0;
+    for (int g = 0; g < ngroups; ++g) {
+      const uint64_t j = o ^ groups[g].xm;
+      double fr = 0.0, fi = 0.0;
+      const uint32_t t0 = groups[g].first, t1 = t0 + groups[g].count;
+      for (uint32_t t = t0; t < t1; ++t) {
+        const bool neg = __popcll(j & terms[t].zm) & 1;
+        fr += neg ? -terms[t].cr : terms[t].cr;
+        fi += neg ? -terms[t].ci : terms[t].ci;
+      }
+      const double pr = (double)psi[j].x, pi = (double)psi[j].y;
+      yr += fr * pr - fi * pi;
+      yi += fr * pi + fi * pr;
+    }
+    acc += (double)psi[o].x * yr + (double)psi[o].y * yi;
+  }
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
 // Partial sums of Re sum_j conj(a_j) b_j, one per workgroup.
@@ -995,25 +971,6 @@ hipError_t launch_norm_partial(int prec, const void* psi, uint64_t dim, double* 
   return hipGetLastError();
 }
 
-hipError_t launch_pauli_partial(int prec, const void* phi, const void* psi, uint64_t dim, uint64_t xm,
-                                uint64_t zm, double2* part, int nblocks, hipStream_t s) {
-  if (prec == 0)
-    dev::k_pauli_partial<double><<<nblocks, 256, 0, s>>>((const double2*)phi, (const double2*)psi, dim, xm, zm, part);
-  else
-    dev::k_pauli_partial<float><<<nblocks, 256, 0, s>>>((const float2*)phi, (const float2*)psi, dim, xm, zm, part);
-  return hipGetLastError();
-}
-
-hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const uint64_t* zmasks,
-                                const double* coeffs, int nterms, double* part, int nblocks,
-                                hipStream_t s) {
-  if (prec == 0)
-    dev::k_zdiag_partial<double><<<nblocks, 256, 0, s>>>((const double2*)psi, dim, zmasks, coeffs, nterms, part);
-  else
-    dev::k_zdiag_partial<float><<<nblocks, 256, 0, s>>>((const float2*)psi, dim, zmasks, coeffs, nterms, part);
-  return hipGetLastError();
-}
-
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s) {
   if (prec == 0)
@@ -1030,6 +987,15 @@ hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, 
     dev::k_pauli_sum<double><<<g, 256, 0, s>>>((double2*)lam, (const double2*)psi, dim, groups, ngroups, terms);
   else
     dev::k_pauli_sum<float><<<g, 256, 0, s>>>((float2*)lam, (const float2*)psi, dim, groups, ngroups, terms);
+  return hipGetLastError();
+}
+
+hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
+                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s) {
+  if (prec == 0)
+    dev::k_pauli_expect<double><<<nblocks, 256, 0, s>>>((const double2*)psi, dim, groups, ngroups, terms, part);
+  else
+    dev::k_pauli_expect<float><<<nblocks, 256, 0, s>>>((const float2*)psi, dim, groups, ngroups, terms, part);
   return hipGetLastError();
 }
 

@@ -100,11 +100,6 @@ hipError_t launch_sample_blocks(int prec, const void* psi, uint64_t dim, int bb,
                                 uint64_t njobs, const double* u, uint64_t* out, hipStream_t s);
 hipError_t launch_norm_partial(int prec, const void* psi, uint64_t dim, double* part, int nblocks,
                                hipStream_t s);
-hipError_t launch_pauli_partial(int prec, const void* phi, const void* psi, uint64_t dim, uint64_t xm,
-                                uint64_t zm, double2* part, int nblocks, hipStream_t s);
-hipError_t launch_zdiag_partial(int prec, const void* psi, uint64_t dim, const uint64_t* zmasks,
-                                const double* coeffs, int nterms, double* part, int nblocks,
-                                hipStream_t s);
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,
                               double2* part, int nblocks, hipStream_t s);
 // One Pauli-sum term of an X-mask group: coefficient (with i^ny folded in) times
@@ -122,6 +117,9 @@ struct PauliGroup {
 // (the whole observable in one pass).
 hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
                             const PauliZTerm* terms, hipStream_t s);
+// part[b] = partial sums of Re <psi| O |psi> over nblocks blocks, O as in launch_pauli_sum.
+hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
+                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s);
 // part[b] = partial sums of Re sum_j conj(a_j) b_j over nblocks blocks.
 hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
                                  hipStream_t s);

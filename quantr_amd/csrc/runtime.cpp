@@ -460,71 +460,26 @@ std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o) {
   return out;
 }
 
-// <psi|O|psi>
+// <psi|O|psi> in one pass over the state (terms grouped by X mask), partials summed on the host.
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms) {
-  const int nb = 512;
-  std::vector<uint64_t> zm;
-  std::vector<double> zc;
-  std::vector<size_t> offdiag;
-  for (size_t t = 0; t < terms.size(); ++t) {
-    if (terms[t].xm == 0) {
-      zm.push_back(terms[t].zm);
-      zc.push_back(terms[t].coeff * ((terms[t].ny & 1) ? 0.0 : 1.0));
-    } else {
-      offdiag.push_back(t);
-    }
-  }
-  const size_t nparts = (zm.empty() ? 0 : nb) + offdiag.size() * nb * 2;
-  std::vector<uint8_t> blob(zm.size() * 8 + zc.size() * 8);
-  if (!zm.empty()) {
-    std::memcpy(blob.data(), zm.data(), zm.size() * 8);
-    std::memcpy(blob.data() + zm.size() * 8, zc.data(), zc.size() * 8);
-  }
+  if (terms.empty()) return 0.0;
+  std::vector<PauliGroup> groups;
+  std::vector<PauliZTerm> flat;
+  pauli_tables(terms, groups, flat);
+  const size_t gb = groups.size() * sizeof(PauliGroup), tb = flat.size() * sizeof(PauliZTerm);
+  std::vector<uint8_t> blob(gb + tb);
+  std::memcpy(blob.data(), groups.data(), gb);
+  if (tb) std::memcpy(blob.data() + gb, flat.data(), tb);
   uint8_t* d = upload(st, st->aux, blob);
-  st->aux2.reserve(std::max<size_t>(nparts, 1) * sizeof(double));
-  double* parts = (double*)st->aux2.p;
-  size_t at = 0;
-  if (!zm.empty()) {
-    hip_check(launch_zdiag_partial(st->prec, st->amp, st->dim, (const uint64_t*)d, (const double*)(d + zm.size() * 8),
-                                   (int)zm.size(), parts, nb, st->stream),
-              "zdiag");
-    at += nb;
-  }
-  for (size_t i = 0; i < offdiag.size(); ++i) {
-    const auto& tm = terms[offdiag[i]];
-    hip_check(launch_pauli_partial(st->prec, st->amp, st->amp, st->dim, tm.xm, tm.zm, (double2*)(parts + at), nb,
-                                   st->stream),
-              "pauli");
-    at += 2 * nb;
-  }
-  std::vector<double> h(nparts);
-  if (nparts)
-    hip_check(hipMemcpyAsync(h.data(), parts, nparts * sizeof(double), hipMemcpyDeviceToHost, st->stream), "D2H");
+  const int nb = (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, st->dim / 256));
+  st->aux2.reserve(nb * sizeof(double));
+  hip_check(launch_pauli_expect(st->prec, st->amp, st->dim, (const PauliGroup*)d, (int)groups.size(),
+                                (const PauliZTerm*)(d + gb), (double*)st->aux2.p, nb, st->stream),
+            "expectation");
+  std::vector<double> h(nb);
+  hip_check(hipMemcpyAsync(h.data(), st->aux2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st->stream), "D2H");
   hip_check(hipStreamSynchronize(st->stream), "sync");
-  double total = 0.0;
-  size_t rd = 0;
-  if (!zm.empty()) {
-    total += sum_partials_host(std::vector<double>(h.begin(), h.begin() + nb));
-    rd = nb;
-  }
-  for (size_t i = 0; i < offdiag.size(); ++i) {
-    const auto& tm = terms[offdiag[i]];
-    std::vector<double> re(nb), im(nb);
-    for (int b = 0; b < nb; ++b) {
-      re[b] = h[rd + 2 * b];
-      im[b] = h[rd + 2 * b + 1];
-    }
-    rd += 2 * nb;
-    double sr = sum_partials_host(re), si = sum_partials_host(im);
-    // multiply by i^ny, keep the real part
-    for (int q = 0; q < (tm.ny & 3); ++q) {
-      const double t = sr;
-      sr = -si;
-      si = t;
-    }
-    total += tm.coeff * sr;
-  }
-  return total;
+  return sum_partials_host(h);
 }
 
 
