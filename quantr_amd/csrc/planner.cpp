@@ -168,20 +168,31 @@ bool independent(const std::vector<uint16_t>& pat, int m, int bits) {
   return rank == m;
 }
 
-// LDS address pattern per tile position: S[p] = (1 << p) ^ e_p with e_p in the low 4 bits
-// for p >= 4 (an invertible linear map). Chosen so that lane slots 0..3 of both maps land on
-// distinct 16-byte slots (low 3 and low 4 address bits independent).
+// True if the rows r[0..q) (bits 0..q-1) are linearly independent over GF(2).
+bool invertible_low(const std::vector<uint16_t>& S, int q) {
+  std::vector<uint16_t> rows(S.begin(), S.begin() + q);
+  for (auto& r : rows) r &= (uint16_t)((1u << q) - 1);
+  return independent(rows, q, q);
+}
+
+// LDS address pattern per tile position (the LDS slot of an amplitude is the XOR of the
+// patterns of its set tile coordinates). 16-byte accesses are served eight lanes at a time:
+// the lanes of each group of eight must hit distinct slots mod 8, i.e. the patterns of thread
+// slots 0..2 (lane bits 0..2) must be independent in the low 3 bits, for the map written
+// (old) and the map read (new). Patterns are S[p] = (1 << p) ^ e_p for p >= 4 and free rows
+// of the low 4 x 4 block for p < 4 (kept invertible, so the map stays a bijection); only the
+// positions held by those lane slots are changed from the identity.
 std::vector<uint16_t> choose_swizzle(int k, int nts, const std::vector<int>& oldm, const std::vector<int>& newm) {
   std::vector<uint16_t> S(k);
   for (int p = 0; p < k; ++p) S[p] = (uint16_t)(1u << p);
-  const int lanes = std::min(nts, 4);
-  if (lanes == 0 || k <= 4) return S;
+  const int lanes = std::min(nts, 3);
+  if (lanes == 0 || k < 3) return S;
+  const int q = std::min(k, 4);
   auto ok = [&]() {
     for (const std::vector<int>* m : {&oldm, &newm}) {
       std::vector<uint16_t> pat(lanes);
       for (int i = 0; i < lanes; ++i) pat[i] = S[(*m)[i]];
-      if (!independent(pat, std::min(lanes, 3), 3)) return false;
-      if (!independent(pat, lanes, 4)) return false;
+      if (!independent(pat, lanes, 3)) return false;
     }
     return true;
   };
@@ -189,14 +200,15 @@ std::vector<uint16_t> choose_swizzle(int k, int nts, const std::vector<int>& old
   std::vector<int> free_pos;
   for (const std::vector<int>* m : {&oldm, &newm})
     for (int i = 0; i < lanes; ++i)
-      if ((*m)[i] >= 4 && std::find(free_pos.begin(), free_pos.end(), (*m)[i]) == free_pos.end())
-        free_pos.push_back((*m)[i]);
+      if (std::find(free_pos.begin(), free_pos.end(), (*m)[i]) == free_pos.end()) free_pos.push_back((*m)[i]);
   uint64_t x = 0x9E3779B97F4A7C15ull;
-  for (int attempt = 0; attempt < 4096; ++attempt) {
+  for (int attempt = 0; attempt < 8192; ++attempt) {
     for (int p : free_pos) {
       x ^= x << 13; x ^= x >> 7; x ^= x << 17;
-      S[p] = (uint16_t)((1u << p) ^ (x & 15));
+      const uint16_t e = (uint16_t)(x & ((1u << q) - 1));
+      S[p] = p >= q ? (uint16_t)((1u << p) ^ e) : e;
     }
+    if (!invertible_low(S, q)) continue;
     if (ok()) return S;
   }
   for (int p = 0; p < k; ++p) S[p] = (uint16_t)(1u << p);
