@@ -410,7 +410,7 @@ def main():
         "tile_sweeps_per_step": r["plan"]["tile_sweeps"],
         "algorithmic_bytes_per_step": r["step_bytes"],
         "norm_error": r["norm_err"],
-        "roofline": {"bound": "hbm", "kernel": "k_tile (fused tile sweep)",
+        "roofline": {"bound": "hbm", "kernel": "qr_stage (compiled fused tile sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch_bytes},

@@ -61,7 +61,7 @@ def stats(csv_path, out, cmd):
         rows = stats_from_trace(csv_path)
     else:
         rows = list(csv.DictReader(open(csv_path)))
-    lines = ["# rocprofv3 --kernel-trace --stats", "", "Command (from the repository root on the GPU host):", "",
+    lines = ["# rocprofv3 kernel durations", "", "Command (from the repository root on the GPU host):", "",
              "    " + cmd, "", "| kernel | calls | avg ms | min ms | max ms | share % |", "|---|---|---|---|---|---|"]
     for r in rows:
         name = r["Name"] if len(r["Name"]) < 70 else r["Name"][:67] + "..."
