@@ -100,13 +100,31 @@ def bench_simulate(args):
     prof = st.profile_read()
     st.profile_enable(False)
     norm = st.norm_sqr()
+    # measure_all on the simulated state: 500 shots (the reference's test size) and 100,000
+    sampling = {}
+    for shots in (500, 100000):
+        u = np.array(W.uniforms(shots, 4242))
+        st.sample(u)  # untimed first call (buffers)
+        st.profile_enable(True)
+        ts0 = time.perf_counter()
+        idx = st.sample(u)
+        t_s = time.perf_counter() - ts0
+        sp = st.profile_read()
+        st.profile_enable(False)
+        bm = sp.get("block_mass", {"total_ms": 0.0, "bytes": 0.0, "launches": 0})
+        sampling[str(shots)] = {"ms": round(t_s * 1e3, 3), "shots_per_s": round(shots / t_s, 1),
+                                "block_mass_ms": round(bm["total_ms"], 3),
+                                "block_mass_gbs": round(bm["bytes"] / max(bm["total_ms"], 1e-9) / 1e6, 1),
+                                "none_shots": int(np.sum(idx == np.iinfo(np.uint64).max))}
+    log("bench: measure_all 500 shots %.2f ms, 100000 shots %.2f ms (block masses at %.0f GB/s)"
+        % (sampling["500"]["ms"], sampling["100000"]["ms"], sampling["500"]["block_mass_gbs"]))
     st.close()
     ms = (t1 - t0) * 1000.0 / args.steps
     init_bytes = float(1 << n) * B
     step_bytes = plan["bytes"] + init_bytes
     log("bench: %.3f ms/step, %d sweeps (%d tile), norm-1 = %.2e" % (ms, plan["sweeps"], plan["tile_sweeps"], norm - 1))
     return {"ms": ms, "plan": plan, "prof": prof, "step_bytes": step_bytes, "gates": len(gates), "n": n,
-            "B": B, "norm_err": abs(norm - 1.0)}
+            "B": B, "norm_err": abs(norm - 1.0), "sampling": sampling}
 
 
 def bench_adjoint(args):
@@ -410,6 +428,7 @@ def main():
         "tile_sweeps_per_step": r["plan"]["tile_sweeps"],
         "algorithmic_bytes_per_step": r["step_bytes"],
         "norm_error": r["norm_err"],
+        "measure_all": r["sampling"],
         "roofline": {"bound": "hbm", "kernel": "qr_stage (compiled fused tile sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,

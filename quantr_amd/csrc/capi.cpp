@@ -233,7 +233,10 @@ qr_status qr_sample(qr_state* st, const double* uniforms, uint64_t shots, uint64
       const int bb = std::min<int>(14, (int)st->n);
       const uint64_t nblk = st->dim >> bb;
       st->aux2.reserve(nblk * 8);
-      hip_check(launch_block_mass(st->prec, st->amp, st->dim, bb, (double*)st->aux2.p, st->stream), "block mass");
+      {
+        ProfScope ps_(st, "block_mass", (double)st->dim * st->bytes_per_amp());
+        hip_check(launch_block_mass(st->prec, st->amp, st->dim, bb, (double*)st->aux2.p, st->stream), "block mass");
+      }
       std::vector<double> mass(nblk);
       hip_check(hipMemcpyAsync(mass.data(), st->aux2.p, nblk * 8, hipMemcpyDeviceToHost, st->stream), "D2H");
       hip_check(hipStreamSynchronize(st->stream), "sync");
@@ -259,6 +262,7 @@ qr_status qr_sample(qr_state* st, const double* uniforms, uint64_t shots, uint64
       uint8_t* d = upload(st, st->aux, blob);
       st->aux2.reserve(shots * 8);
       hip_check(hipMemsetAsync(st->aux2.p, 0xff, shots * 8, st->stream), "memset");
+      ProfScope ps_(st, "sample_blocks", (double)jobs.size() * (double)(1ull << bb) * st->bytes_per_amp());
       hip_check(launch_sample_blocks(st->prec, st->amp, st->dim, bb, (const BlockJob*)(d + shots * 8), jobs.size(),
                                      (const double*)d, (uint64_t*)st->aux2.p, st->stream),
                 "sample blocks");
