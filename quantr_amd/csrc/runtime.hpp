@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <string>
@@ -141,7 +142,7 @@ struct CachedPlan {
   std::vector<Stage> stages;
   std::vector<std::string> src;    // compiled-stage sources (filled on first compiled use)
   std::vector<hipFunction_t> fns;  // their kernels
-  uint64_t runs = 0;
+  std::atomic<uint64_t> runs{0};  // applications (states on several threads may share a plan)
   uint64_t stamp = 0;
 };
 
