@@ -367,17 +367,18 @@ struct Group {
 
 // Compiles todo[i] for every i of every group; returns the groups that produced a code object.
 void compile_in_children(const std::vector<std::string>& todo, std::vector<Group>& groups, const std::string& cc) {
-  char tmpl[] = "/tmp/qr_stages_XXXXXX";
+  // a private directory under $TMPDIR, else /tmp
+  auto make_dir = [](const std::string& parent, std::string& out) {
+    std::string t = parent + "/qr_stages_XXXXXX";
+    std::vector<char> buf(t.begin(), t.end());
+    buf.push_back('\0');
+    if (!mkdtemp(buf.data())) return false;
+    out = buf.data();
+    return true;
+  };
   const char* td = std::getenv("TMPDIR");
-  std::string base = std::string(td && *td ? td : "/tmp") + "/qr_stages_XXXXXX";
-  std::vector<char> dir(base.begin(), base.end());
-  dir.push_back('\0');
-  if (!mkdtemp(dir.data())) {
-    std::memcpy(dir.data(), tmpl, sizeof tmpl);
-    dir.resize(sizeof tmpl);
-    if (!mkdtemp(dir.data())) return;
-  }
-  const std::string d(dir.data());
+  std::string d;
+  if (!(td && *td && make_dir(td, d)) && !make_dir("/tmp", d)) return;
   for (size_t g = 0; g < groups.size(); ++g) {
     Group& G = groups[g];
     G.src = d + "/g" + std::to_string(g) + ".hip";
@@ -413,7 +414,6 @@ void compile_in_children(const std::vector<std::string>& todo, std::vector<Group
     (void)unlink(G.src.c_str());
     (void)unlink(G.log.c_str());
   }
-  (void)d;
 }
 
 }  // namespace
