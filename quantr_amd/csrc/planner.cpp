@@ -125,6 +125,72 @@ uint64_t greedy_phase_bits(int want, const std::vector<int>& cands, const std::v
   return cur;
 }
 
+// Register set for the next phase, with one phase of lookahead: every `want`-subset of
+// `cands` is scored by what it absorbs (phase_score); the best few are scored again by what
+// the best following subset of `next_cands` then absorbs. A set that absorbs everything left
+// wins outright, preferring one without bits of `low` (the stage then ends without a remap
+// back to the coalesced store layout). Each remap costs a round trip of the tile through LDS,
+// so fewer phases is the objective; the lookahead avoids greedy sets that strand ops.
+uint64_t lookahead_phase_bits(int want, const std::vector<int>& cands, const std::vector<int>& next_cands,
+                              const std::vector<int>& remaining, const std::vector<Masks>& mk,
+                              const std::vector<uint8_t>& eligible, const std::vector<uint64_t>& cmask,
+                              uint64_t tile, uint64_t low) {
+  const int nc = (int)cands.size();
+  want = std::min(want, nc);
+  auto subsets = [](const std::vector<int>& cs, int w) {
+    std::vector<uint64_t> out;
+    const int m = (int)cs.size();
+    if (w <= 0 || w > m || m > 20) return out;
+    for (uint32_t s = 0; s < (1u << m); ++s) {
+      if (__builtin_popcount(s) != w) continue;
+      uint64_t b = 0;
+      for (int j = 0; j < m; ++j)
+        if (s & (1u << j)) b |= 1ull << cs[j];
+      out.push_back(b);
+    }
+    return out;
+  };
+  const std::vector<uint64_t> first = subsets(cands, want);
+  const std::vector<uint64_t> second = subsets(next_cands, std::min<int>(want, (int)next_cands.size()));
+  if (first.empty()) return greedy_phase_bits(want, cands, remaining, mk, eligible, cmask, tile);
+  const int total = (int)remaining.size();
+  std::vector<std::pair<int, uint64_t>> scored;
+  scored.reserve(first.size());
+  for (uint64_t s : first) scored.push_back({phase_score(remaining, mk, eligible, cmask, s, tile), s});
+  std::stable_sort(scored.begin(), scored.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  // a set that finishes the stage
+  uint64_t finish = 0;
+  bool have_finish = false;
+  for (const auto& sc : scored) {
+    if (count_absorb(remaining, mk, eligible, sc.second) != total) continue;
+    if (!have_finish || ((finish & low) && !(sc.second & low))) finish = sc.second;
+    have_finish = true;
+  }
+  if (have_finish) return finish;
+  const int keep = std::min<int>((int)scored.size(), 24);
+  int best = -1;
+  uint64_t best_set = scored[0].second;
+  std::vector<int> left;
+  for (int i = 0; i < keep; ++i) {
+    if (scored[i].first <= 0) break;
+    left.clear();
+    absorb(remaining, mk, eligible, scored[i].second, nullptr, &left);
+    int s2 = 0;
+    for (uint64_t t : second) {
+      int v = phase_score(left, mk, eligible, cmask, t, tile);
+      // finishing after the next phase, without a final remap, is worth one more op
+      if (count_absorb(left, mk, eligible, t) == (int)left.size()) v += (t & low) ? 2 : 6;
+      s2 = std::max(s2, v);
+    }
+    const int v = scored[i].first + s2;
+    if (v > best) {
+      best = v;
+      best_set = scored[i].second;
+    }
+  }
+  return best_set;
+}
+
 // Greedy choice of `want` extra bits from `cands` on top of `base` maximising absorption.
 uint64_t greedy_bits(uint64_t base, int want, const std::vector<int>& cands,
                      const std::vector<int>& remaining, const std::vector<Masks>& mk,
@@ -330,7 +396,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
 
     std::vector<int> rem = taken;
     // first register set: from the non-low tile bits so the load stays coalesced
-    uint64_t R = greedy_phase_bits(r, nonlow_bits, rem, mk, eligible, cmask, B);
+    uint64_t R = lookahead_phase_bits(r, nonlow_bits, tile_bits_vec, rem, mk, eligible, cmask, B, low_mask);
     for (int b : nonlow_bits)
       if (__builtin_popcountll(R) < r) R |= 1ull << b;
     map = make_map(R);
@@ -477,7 +543,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       rem = later;
       if (rem.empty() || hit_swap) continue;
       // next register set: any tile bits
-      uint64_t R2 = greedy_phase_bits(r, tile_bits_vec, rem, mk, eligible, cmask, B);
+      uint64_t R2 = lookahead_phase_bits(r, tile_bits_vec, tile_bits_vec, rem, mk, eligible, cmask, B, low_mask);
       for (int b : tile_bits_vec)
         if (__builtin_popcountll(R2) < r) R2 |= 1ull << b;
       if (now.empty() && count_absorb(rem, mk, eligible, R2) == 0)
