@@ -43,6 +43,7 @@ def parse():
     ap.add_argument("--low-bits", type=int, default=0)
     ap.add_argument("--tile-variant", type=int, default=0)
     ap.add_argument("--reg-bits", type=int, default=0)
+    ap.add_argument("--stage-ops", type=int, default=0, help="gates per tile sweep at most (0 = auto)")
     ap.add_argument("--jit", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -80,7 +81,7 @@ def bench_simulate(args):
     prog = W.to_program(gates).c_struct()
     st = _capi.State(n, prec)
     st.set_options(fused=1, tile_bits=args.tile_bits, low_bits=args.low_bits, tile_variant=args.tile_variant,
-                   reg_bits=args.reg_bits, jit=args.jit)
+                   reg_bits=args.reg_bits, jit=args.jit, stage_ops=args.stage_ops)
     log("bench: %d-qubit %s brickwork depth %d, %d gates" % (n, args.precision, args.depth, len(gates)))
     for i in range(args.warmup):
         t0 = time.perf_counter()

@@ -149,6 +149,9 @@ typedef struct qr_options {
                             0 never; 1 always; 2 (default) from the second application of
                             the same circuit structure in this process, or at once for
                             n >= 28. Plans are cached per structure either way.          */
+  uint32_t stage_ops;    /* gates fused into one tile sweep at most (0 = auto). A capped
+                            stage leaves its tail to later sweeps, which balances ALU work
+                            between sweeps; results do not depend on it.                 */
 } qr_options;
 
 typedef struct qr_state qr_state;

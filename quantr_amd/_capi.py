@@ -44,7 +44,7 @@ class QrObservable(C.Structure):
 class QrOptions(C.Structure):
     _fields_ = [("fused", C.c_uint32), ("tile_bits", C.c_uint32), ("low_bits", C.c_uint32),
                 ("exact_sample", C.c_uint32), ("tile_variant", C.c_uint32), ("reg_bits", C.c_uint32),
-                ("jit", C.c_uint32)]
+                ("jit", C.c_uint32), ("stage_ops", C.c_uint32)]
 
 
 class QrBatchOptions(C.Structure):
@@ -254,8 +254,9 @@ class State:
     def handle(self):
         return self._h
 
-    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2):
-        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit)
+    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2,
+                    stage_ops=0):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops)
         check(lib().qr_state_set_options(self._h, C.byref(o)))
 
     def reset(self):
@@ -342,8 +343,9 @@ class Grad:
         check(lib().qr_grad_forward(self._h, C.byref(p), C.byref(o), C.byref(v)))
         return v.value
 
-    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2):
-        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit)
+    def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2,
+                    stage_ops=0):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops)
         check(lib().qr_grad_set_options(self._h, C.byref(o)))
 
     def num_params(self):

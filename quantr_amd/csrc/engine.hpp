@@ -162,6 +162,13 @@ constexpr int kWholeStateStageOps = 96;
 
 std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
                               int bytes_per_amp, bool dual);
+// States of at least 2^kBalanceMinQubits amplitudes get balanced forward plans.
+constexpr uint32_t kBalanceMinQubits = 25;
+// plan_fused with the ops per stage chosen by estimated duration (HBM vs ALU time per stage)
+// among a few caps, for forward plans of large states without a caller-set cap; otherwise
+// plan_fused itself.
+std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
+                                       int bytes_per_amp, bool dual);
 // Appends what the planner's output depends on for one op (kind, bits, controls and the
 // value-derived forms), so that programs with equal keys share one plan; coefficients are
 // then taken from each program's own ops (TileOpHost::src).

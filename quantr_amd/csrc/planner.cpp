@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <exception>
+#include <thread>
 
 #include "engine.hpp"
 
@@ -652,6 +654,73 @@ void stage_coefficients(const Stage& s, const std::vector<FastOp>& ops, std::vec
   if (absorber >= 0)
     for (int q = 0; q < 4; ++q) c[4 * absorber + q] *= total;
   if (scaler >= 0) c[4 * scaler] = total;
+}
+
+// ---- stage balancing ----------------------------------------------------------------------
+//
+// A tile sweep takes the longer of its HBM time and its ALU time. The greedy planner fills
+// each stage with every op it can take, so some stages carry 170 ops (ALU-bound, twice the
+// HBM time) while others carry 20. Capping the ops per stage moves work into later, lighter
+// sweeps; whether a cap adds sweeps or even removes some depends on the circuit, so a few
+// caps are planned and the one with the lowest estimated duration is kept.
+namespace {
+
+// Ops (plus LDS remaps) whose ALU time over a complex128 state equals one sweep of it at copy
+// speed: measured on MI355X, where a plain copy moves about 5.4 TB/s and a 172-op stage of
+// the 30-qubit brickwork takes 2.1 times a copy-speed sweep. Proportional to the bytes per
+// amplitude, since the ALU work per amplitude does not depend on the precision.
+constexpr double kSweepOpsC128 = 87.0;
+
+double plan_cost(const std::vector<Stage>& st, int bytes_per_amp) {
+  const double sweep = kSweepOpsC128 * bytes_per_amp / 16.0;
+  double cost = 0.0;
+  for (const Stage& s : st) {
+    if (!s.tile) {
+      cost += sweep;
+      continue;
+    }
+    double units = s.ngates;
+    for (const auto& t : s.ops)
+      if (t.code == TC_REMAP) units += 1.0;
+    cost += std::max(sweep, units);
+  }
+  return cost;
+}
+
+}  // namespace
+
+std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
+                                       int bytes_per_amp, bool dual) {
+  if (pp.max_ops > 0 || dual || n < kBalanceMinQubits || pp.tile_bits >= (int)n)
+    return plan_fused(n, ops, pp, bytes_per_amp, dual);
+  static const int caps[] = {0, 160, 140, 120, 100, 90, 80};
+  constexpr int nc = (int)(sizeof(caps) / sizeof(caps[0]));
+  std::vector<std::vector<Stage>> plans(nc);
+  std::vector<std::exception_ptr> errs(nc);
+  std::vector<std::thread> th;
+  for (int i = 0; i < nc; ++i)
+    th.emplace_back([&, i] {
+      try {
+        PlanParams q = pp;
+        q.max_ops = caps[i];
+        plans[i] = plan_fused(n, ops, q, bytes_per_amp, dual);
+      } catch (...) {
+        errs[i] = std::current_exception();
+      }
+    });
+  for (auto& t : th) t.join();
+  for (auto& e : errs)
+    if (e) std::rethrow_exception(e);
+  int best = 0;
+  double best_cost = plan_cost(plans[0], bytes_per_amp);
+  for (int i = 1; i < nc; ++i) {
+    const double c = plan_cost(plans[i], bytes_per_amp);
+    if (c < best_cost) {
+      best_cost = c;
+      best = i;
+    }
+  }
+  return std::move(plans[best]);
 }
 
 }  // namespace qr

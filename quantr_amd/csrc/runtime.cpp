@@ -28,13 +28,21 @@ int pick_tile_bits(const qr_state* st) {
   return std::min<int>(k, (int)st->n);
 }
 
+// Ops per tile stage at most: the caller's stage_ops, else kWholeStateStageOps when one tile
+// covers the whole state, else no limit.
+int stage_op_limit(const qr_state* st, int tile_bits) {
+  int lim = tile_bits == (int)st->n ? kWholeStateStageOps : 0;
+  if (st->opt.stage_ops) lim = lim ? std::min<int>(lim, (int)st->opt.stage_ops) : (int)st->opt.stage_ops;
+  return lim;
+}
+
 PlanParams plan_params(const qr_state* st) {
   PlanParams pp;
   pp.tile_bits = pick_tile_bits(st);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);
   if ((st->opt.reg_bits == 5 || st->opt.reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = (int)st->opt.reg_bits;
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
-  if (pp.tile_bits == (int)st->n) pp.max_ops = kWholeStateStageOps;
+  pp.max_ops = stage_op_limit(st, pp.tile_bits);
   return pp;
 }
 
@@ -211,7 +219,7 @@ std::shared_ptr<CachedPlan> get_plan_for(uint32_t n, int prec, const std::vector
   auto cp = std::make_shared<CachedPlan>();
   const int B = prec == 0 ? 16 : 8;
   if (fused) {
-    cp->stages = plan_fused(n, ops, pp, B, dual);
+    cp->stages = plan_fused_balanced(n, ops, pp, B, dual);
   } else {
     for (size_t i = 0; i < ops.size(); ++i) {
       Stage s;
@@ -532,7 +540,7 @@ PlanParams dual_plan_params(const qr_state* st) {
   pp.tile_bits = std::min<int>(k, (int)st->n);
   pp.reg_bits = std::min(4, pp.tile_bits);
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
-  if (pp.tile_bits == (int)st->n) pp.max_ops = kWholeStateStageOps;
+  pp.max_ops = stage_op_limit(st, pp.tile_bits);
   return pp;
 }
 

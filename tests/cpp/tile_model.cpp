@@ -220,6 +220,32 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
   }
 }
 
+// model_run with a cap on the ops per tile stage (the planner's max_ops), register bits 4.
+int model_run_capped(uint32_t n, double* state, const qr_program* prog, int tile_bits, int low_bits, int max_ops,
+                     uint64_t* sweeps_out) {
+  try {
+    auto ops = lower_fast(n, lower_exact(n, prog), false);
+    PlanParams pp;
+    pp.tile_bits = std::min<int>(tile_bits, (int)n);
+    pp.reg_bits = std::min(4, pp.tile_bits);
+    pp.low_bits = low_bits;
+    pp.max_ops = max_ops;
+    auto stages = plan_fused(n, ops, pp, 16, false);
+    std::vector<cdv> psi(1ull << n);
+    std::memcpy(psi.data(), state, psi.size() * 16);
+    for (const Stage& s : stages) {
+      if (s.tile) run_tile(n, psi, s, pp.reg_bits);
+      else apply_standalone(n, psi, ops[s.standalone]);
+    }
+    std::memcpy(state, psi.data(), psi.size() * 16);
+    if (sweeps_out) *sweeps_out = stages.size();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return 1;
+  }
+}
+
 // The adjoint sweep on the CPU model: psi = U|0>, lam = O psi given; plans the reverse
 // program with the engine's dual-state planner and replays it. grads: one per parameter.
 int model_adjoint(uint32_t n, double* psi_io, double* lam_io, const qr_program* prog, int tile_bits, int low_bits,

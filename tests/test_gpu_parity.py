@@ -191,6 +191,30 @@ def test_brickwork_20q_vs_oracle():
     assert plan["sweeps"] < 120
 
 
+def test_balanced_plan_25q_vs_exact():
+    """From 2^25 amplitudes, forward plans are stage-balanced (several ops-per-stage caps are
+    planned and the lowest estimated duration kept); a caller-set cap is honoured as given.
+    Both match the reference-order mode."""
+    n = 25
+    gates = W.brickwork(n, depth=24, seed=77)
+    p = W.to_program(gates)
+    say("balanced 25q exact")
+    want = run_gpu(n, p, fused=0)
+    for jit in (0, 1):
+        say("balanced 25q fused jit=%d" % jit)
+        got = run_gpu(n, p, fused=1, jit=jit)
+        assert np.max(np.abs(got - want)) < TOL_FUSED, jit
+    say("balanced 25q stage_ops=7")
+    st = _capi.State(n)
+    st.set_options(fused=1, jit=0, stage_ops=7)
+    st.apply(p)
+    got = st.amplitudes()
+    plan = st.last_plan()
+    st.close()
+    assert np.max(np.abs(got - want)) < TOL_FUSED
+    assert plan["sweeps"] >= len(gates) // 7
+
+
 def test_norm_and_expectation():
     rng = np.random.default_rng(77)
     n = 12

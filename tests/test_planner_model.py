@@ -85,6 +85,29 @@ def test_model_whole_state_stage_limit():
     assert sweeps >= 4
 
 
+@pytest.mark.parametrize("cap", [1, 3, 9, 40])
+def test_model_stage_caps(cap):
+    """Capped stages (the planner's ops-per-stage limit, which stage balancing varies) defer
+    their tail to later sweeps without reordering dependent ops."""
+    lib = model()
+    lib.model_run_capped.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int,
+                                     C.POINTER(C.c_uint64)]
+    rng = np.random.default_rng(300 + cap)
+    for trial in range(6):
+        n = int(rng.integers(7, 12))
+        gates, customs = H.random_gates(n, 120, rng, with_custom=(trial % 2 == 0))
+        p = H.to_program(gates, customs)
+        psi = np.zeros(1 << n, dtype=np.complex128)
+        psi[0] = 1.0
+        sw = C.c_uint64()
+        ps = p.c_struct()
+        rc = lib.model_run_capped(n, psi.ctypes.data, C.byref(ps), 6, 3, cap, C.byref(sw))
+        assert rc == 0, lib.model_error()
+        assert np.max(np.abs(psi - H.oracle_simulate(n, p))) < 1e-12, (cap, trial)
+        if cap == 1:
+            assert sw.value >= len(gates) // 2
+
+
 def test_model_swap_heavy():
     rng = np.random.default_rng(44)
     for trial in range(10):
