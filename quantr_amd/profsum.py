@@ -3,6 +3,7 @@
 Usage:
     python -m quantr_amd.profsum stats  <run_kernel_stats.csv> <out.md> <command...>
     python -m quantr_amd.profsum traffic <FETCH_SIZE csv> <WRITE_SIZE csv> <kernel> <bytes> <out> <command...>
+    python -m quantr_amd.profsum valu <counter csv> <kernel> <out.md> <command...>
 
 `traffic` scales both counters with kernels of known traffic that the same process ran:
 `k_norm_partial` reads the whole state once (FETCH_SIZE calibration) and `k_init_basis`
@@ -97,11 +98,50 @@ def traffic(fetch_csv, write_csv, kernel, state_bytes, out, cmd):
     return rec
 
 
+def valu(counter_csv, kernel, out, cmd, simds=1024, xcds=8):
+    """Per-dispatch VALU counters of one kernel kind. SQ_INSTS_VALU counts wave instructions;
+    a wave64 VALU instruction holds a 16-lane SIMD for 4 cycles, so VALU busy = 4 * insts /
+    (SIMDs * cycles), with the kernel's cycles from GRBM_GUI_ACTIVE (summed over the XCDs)."""
+    d = collections.defaultdict(lambda: collections.defaultdict(list))
+    grid = {}
+    with open(counter_csv) as f:
+        for r in csv.DictReader(f):
+            k = kernel_kind(r["Kernel_Name"])
+            d[k][r["Counter_Name"]].append(float(r["Counter_Value"]))
+            g0, wg0, v0 = grid.get(k, (0, 0, set()))
+            v0.add(int(r["VGPR_Count"]))
+            grid[k] = (int(r["Grid_Size"]), int(r["Workgroup_Size"]), v0)
+    c = d[find(d, kernel)]
+    avg = {name: sum(v) / len(v) for name, v in c.items()}
+    g, wg, vgpr = grid[find(d, kernel)]
+    waves = g / 64.0
+    cycles = avg["GRBM_GUI_ACTIVE"] / xcds
+    f64 = sum(avg.get(x, 0.0) for x in ("SQ_INSTS_VALU_FMA_F64", "SQ_INSTS_VALU_ADD_F64", "SQ_INSTS_VALU_MUL_F64"))
+    lines = ["# rocprofv3 VALU counters: %s" % kernel, "", "Command (from the repository root on the GPU host):", "",
+             "    " + cmd, "", "| quantity | per dispatch (average of %d) |" % len(c["SQ_INSTS_VALU"]), "|---|---|"]
+    rows = [("grid threads / workgroup", "%d / %d" % (g, wg)),
+            ("VGPR_Count column as reported (range over dispatches)", "%d-%d" % (min(vgpr), max(vgpr))),
+            ("waves", "%.0f" % waves)]
+    for name in sorted(avg):
+        rows.append((name, "%.4g" % avg[name]))
+    rows += [("VALU instructions per wave", "%.0f" % (avg["SQ_INSTS_VALU"] / waves)),
+             ("FP64 share of VALU instructions", "%.2f" % (f64 / avg["SQ_INSTS_VALU"])),
+             ("LDS instructions per wave", "%.0f" % (avg.get("SQ_INSTS_LDS", 0.0) / waves)),
+             ("kernel cycles per XCD (GRBM_GUI_ACTIVE / %d)" % xcds, "%.4g" % cycles),
+             ("VALU busy = 4 * SQ_INSTS_VALU / (%d SIMDs * cycles)" % simds,
+              "%.3f" % (4.0 * avg["SQ_INSTS_VALU"] / (simds * cycles)))]
+    for a, b in rows:
+        lines.append("| %s | %s |" % (a, b))
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
 if __name__ == "__main__":
     if sys.argv[1] == "stats":
         stats(sys.argv[2], sys.argv[3], " ".join(sys.argv[4:]))
     elif sys.argv[1] == "traffic":
         print(json.dumps(traffic(sys.argv[2], sys.argv[3], sys.argv[4], float(sys.argv[5]), sys.argv[6],
                                  " ".join(sys.argv[7:])), indent=1))
+    elif sys.argv[1] == "valu":
+        valu(sys.argv[2], sys.argv[3], sys.argv[4], " ".join(sys.argv[5:]))
     else:
-        raise SystemExit("usage: profsum stats|traffic ...")
+        raise SystemExit("usage: profsum stats|traffic|valu ...")
