@@ -27,7 +27,8 @@
 extern "C" {
 #endif
 
-#define QR_ABI_VERSION 1u
+/* 2: qr_options gained stage_ops (a struct of 8 words; version 1 had 7). */
+#define QR_ABI_VERSION 2u
 
 typedef enum qr_status {
   QR_OK = 0,

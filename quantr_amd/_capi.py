@@ -106,6 +106,7 @@ _SIGNATURES = [
 EXPORTED = [s[0] for s in _SIGNATURES]
 
 _lib = None
+ABI_VERSION = 2  # QR_ABI_VERSION of include/quantr_hip.h; the structures above follow it
 
 
 class QrError(RuntimeError):
@@ -125,6 +126,9 @@ def lib():
             f = getattr(l, name)
             f.restype = res
             f.argtypes = args
+        if l.qr_abi_version() != ABI_VERSION:
+            raise RuntimeError("%s has C ABI version %d; this binding expects %d (rebuild with "
+                               "__graft_entry__.build())" % (LIB_PATH, l.qr_abi_version(), ABI_VERSION))
         _lib = l
     return _lib
 
