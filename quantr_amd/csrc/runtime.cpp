@@ -30,7 +30,7 @@ int pick_tile_bits(const qr_state* st) {
 
 // Ops per tile stage at most: the caller's stage_ops, else kWholeStateStageOps when one tile
 // covers the whole state, else no limit.
-int stage_op_limit(const qr_state* st, int tile_bits) {
+static int stage_op_limit(const qr_state* st, int tile_bits) {
   int lim = tile_bits == (int)st->n ? kWholeStateStageOps : 0;
   if (st->opt.stage_ops) lim = lim ? std::min<int>(lim, (int)st->opt.stage_ops) : (int)st->opt.stage_ops;
   return lim;
