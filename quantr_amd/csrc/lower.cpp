@@ -395,7 +395,9 @@ void classify(FastOp& f) {
   f.form = U1_GENERAL;
   f.sub = SUB_NONE;
   if (f.kind == FK_DIAG1) {
-    if (!is_one(f.m[0])) f.sub = SUB_D0;
+    // d0 is factored out only when it is the larger entry: diag(1, d1/d0) stays finite and
+    // bounded (a zero or tiny d0, e.g. a post-selection onto |1>, keeps its general form)
+    if (!is_one(f.m[0]) && f.m[0] != cd(0, 0) && std::abs(f.m[0]) >= std::abs(f.m[1])) f.sub = SUB_D0;
     return;
   }
   if (f.kind != FK_DENSE1) return;

@@ -242,3 +242,29 @@ def test_model_adjoint_hea():
     want = shift_gradients(n, gates, {}, obs)
     assert np.max(np.abs(got - want)) < 1e-10
     assert sweeps < len(gates) / 3
+
+
+@pytest.mark.parametrize("diag", [(0.0, np.sqrt(2.0)), (np.sqrt(2.0), 0.0), (0.0, 0.0), (1e-300, 1.0),
+                                  (0.5j, 3.0), (2.0, 0.5)])
+@pytest.mark.parametrize("controls", [(), (0,)])
+def test_model_custom_diagonal_zero_entries(diag, controls):
+    """Custom diagonal gates with zero or tiny entries (post-selection, the zero gate) inside a
+    fused tile stage: no entry is ever divided by a zero or tiny one."""
+    n = 3
+    images = np.zeros((1 << (1 + len(controls)), 1 << (1 + len(controls))), dtype=np.complex128)
+    dim = images.shape[0]
+    for j in range(dim):
+        images[j, j] = 1.0
+    images[dim - 2, dim - 2] = diag[0]
+    images[dim - 1, dim - 1] = diag[1]
+    p = H.Program()
+    for w in range(n):
+        p.add("H", w)
+    p.add_custom(1, list(controls), images, np.ones(dim, dtype=np.uint8))
+    p.add("Rx", 2, (), 0.3)
+    p.add("T", 1)
+    want = H.oracle_simulate(n, p)
+    for tile_bits in (2, 3):
+        got, _ = run_model(n, p, tile_bits)
+        assert np.all(np.isfinite(got)), (diag, controls, tile_bits)
+        assert np.max(np.abs(got - want)) < 1e-12, (diag, controls, tile_bits)
