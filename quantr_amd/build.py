@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["kernels.hip", "lower.cpp", "planner.cpp", "runtime.cpp", "batch.cpp", "capi.cpp", "jit.cpp"]
+SOURCES = ["kernels.hip", "tile_interp.hip", "lower.cpp", "planner.cpp", "runtime.cpp", "batch.cpp", "capi.cpp", "jit.cpp", "sampler.cpp"]
 
 
 def embed_tile_ops():
@@ -48,7 +48,28 @@ def _run(cmd):
     return out
 
 
-def build_engine(verbose=False):
+def _deps_mtime(path, seen=None):
+    """Newest modification time of a source and the project headers it includes (transitively)."""
+    seen = set() if seen is None else seen
+    if path in seen or not os.path.exists(path):
+        return 0.0
+    seen.add(path)
+    t = os.path.getmtime(path)
+    for line in open(path, errors="replace"):
+        line = line.strip()
+        if line.startswith("#include \""):
+            name = line.split("\"")[1]
+            for d in (os.path.dirname(path), CSRC, INCLUDE):
+                cand = os.path.join(d, name)
+                if os.path.exists(cand):
+                    t = max(t, _deps_mtime(cand, seen))
+                    break
+    return t
+
+
+def build_engine(verbose=False, force=True):
+    """Compiles every source (force=True), or only those whose object is older than the source
+    or any header (force=False, for quick rebuilds while developing), and links the library."""
     os.makedirs(BUILD, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     objs = []
@@ -58,9 +79,12 @@ def build_engine(verbose=False):
         path = src if os.path.isabs(src) else os.path.join(CSRC, src)
         obj = os.path.join(BUILD, os.path.basename(src) + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj])
         objs.append(obj)
-    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        if (not force and os.path.exists(obj)
+                and os.path.getmtime(obj) > _deps_mtime(path)):
+            continue
+        cmds.append([HIPCC] + HIP_FLAGS + lang + ["-c", path, "-o", obj])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=8) as ex:
         for out in ex.map(_run, cmds):
             if verbose and out:
                 print(out)
@@ -78,11 +102,11 @@ def build_oracle():
     return os.path.join(odir, "_build", "liboracle.so")
 
 
-def build_all(verbose=False):
-    so = build_engine(verbose)
+def build_all(verbose=False, force=True):
+    so = build_engine(verbose, force)
     orc = build_oracle()
     return so, orc
 
 
 if __name__ == "__main__":
-    print(build_all(verbose=True))
+    print(build_all(verbose=True, force="--changed" not in sys.argv))
