@@ -112,13 +112,16 @@ def bench_simulate(args):
         t_s = time.perf_counter() - ts0
         sp = st.profile_read()
         st.profile_enable(False)
-        bm = sp.get("block_mass", {"total_ms": 0.0, "bytes": 0.0, "launches": 0})
-        sampling[str(shots)] = {"ms": round(t_s * 1e3, 3), "shots_per_s": round(shots / t_s, 1),
-                                "block_mass_ms": round(bm["total_ms"], 3),
-                                "block_mass_gbs": round(bm["bytes"] / max(bm["total_ms"], 1e-9) / 1e6, 1),
-                                "none_shots": int(np.sum(idx == np.iinfo(np.uint64).max))}
-    log("bench: measure_all 500 shots %.2f ms, 100000 shots %.2f ms (block masses at %.0f GB/s)"
-        % (sampling["500"]["ms"], sampling["100000"]["ms"], sampling["500"]["block_mass_gbs"]))
+        rec = {"ms": round(t_s * 1e3, 3), "shots_per_s": round(shots / t_s, 1),
+               "none_shots": int(np.sum(idx == np.iinfo(np.uint64).max))}
+        for k in ("block_mass", "block_exact_sums", "sample_exact_blocks"):
+            e = sp.get(k, {"total_ms": 0.0, "bytes": 0.0, "launches": 0})
+            rec[k + "_ms"] = round(e["total_ms"], 3)
+            rec[k + "_gbs"] = round(e["bytes"] / max(e["total_ms"], 1e-9) / 1e6, 1)
+        sampling[str(shots)] = rec
+    log("bench: measure_all 500 shots %.2f ms, 100000 shots %.2f ms (block masses %.0f GB/s, exact sums %.0f GB/s)"
+        % (sampling["500"]["ms"], sampling["100000"]["ms"], sampling["500"]["block_mass_gbs"],
+           sampling["500"]["block_exact_sums_gbs"]))
     st.close()
     ms = (t1 - t0) * 1000.0 / args.steps
     init_bytes = float(1 << n) * B
@@ -264,23 +267,11 @@ def capacity(args):
     return out
 
 
-def batch_instances(count, seed=512):
-    """count HEA instances: n ~ U{10..24}, L ~ U{2..12}, theta ~ U[-pi, pi), each its own seed."""
-    rng = W.SplitMix64(seed)
-    out = []
-    for _ in range(count):
-        n = 10 + rng.below(15)
-        L = 2 + rng.below(11)
-        gates, thetas = W.hea(n, L, seed=rng.next_u64())
-        out.append((n, L, gates, thetas))
-    return out
-
-
 def bench_batch(args):
     """Batched serving: every instance's expectation value and full adjoint gradient from one
     graph launch (complex64), against the plain GPU parameter-shift path on one parameter per
     instance (complex64 timing, complex128 reference for the error)."""
-    inst = batch_instances(args.batch_instances)
+    inst = W.batch_instances(args.batch_instances)
     P = sum(len(t) for _, _, _, t in inst)
     obs = {n: _capi.Observable(W.hea_observable(n)) for n in range(10, 25)}
     items = [(n, W.to_program(g), obs[n]) for n, _, g, _ in inst]

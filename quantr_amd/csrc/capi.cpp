@@ -220,8 +220,8 @@ qr_status qr_sample(qr_state* st, const double* uniforms, uint64_t shots, uint64
     std::vector<double> us(shots);
     for (uint64_t s = 0; s < shots; ++s) us[s] = uniforms[perm[s]];
     std::vector<uint64_t> res(shots, ~0ull);
-    const bool exact = st->opt.exact_sample == 1 || (st->opt.exact_sample == 2 && st->n <= 26);
-    if (exact) {
+    if (st->opt.exact_sample == 1) {
+      // checker mode: one lane folds the whole state in index order
       std::vector<uint8_t> blob(shots * 8);
       std::memcpy(blob.data(), us.data(), shots * 8);
       uint8_t* d = upload(st, st->aux, blob);
@@ -230,46 +230,11 @@ qr_status qr_sample(qr_state* st, const double* uniforms, uint64_t shots, uint64
       hip_check(launch_sample_fold(st->prec, st->amp, st->dim, (const double*)d, shots, (uint64_t*)st->aux2.p,
                                    st->stream),
                 "sample fold");
-    } else {
-      const int bb = std::min<int>(14, (int)st->n);
-      const uint64_t nblk = st->dim >> bb;
-      st->aux2.reserve(nblk * 8);
-      {
-        ProfScope ps_(st, "block_mass", (double)st->dim * st->bytes_per_amp());
-        hip_check(launch_block_mass(st->prec, st->amp, st->dim, bb, (double*)st->aux2.p, st->stream), "block mass");
-      }
-      std::vector<double> mass(nblk);
-      hip_check(hipMemcpyAsync(mass.data(), st->aux2.p, nblk * 8, hipMemcpyDeviceToHost, st->stream), "D2H");
+      hip_check(hipMemcpyAsync(res.data(), st->aux2.p, shots * 8, hipMemcpyDeviceToHost, st->stream), "D2H");
       hip_check(hipStreamSynchronize(st->stream), "sync");
-      std::vector<double> incl(nblk);
-      double cum = 0.0;
-      for (uint64_t b = 0; b < nblk; ++b) {
-        cum += mass[b];
-        incl[b] = cum;
-      }
-      std::vector<BlockJob> jobs;
-      uint64_t s = 0;
-      while (s < shots) {
-        const uint64_t b = (uint64_t)(std::upper_bound(incl.begin(), incl.end(), us[s]) - incl.begin());
-        if (b >= nblk) break;  // u >= total mass: None for this and all larger draws
-        uint64_t e = s;
-        while (e < shots && us[e] < incl[b]) ++e;
-        jobs.push_back(BlockJob{b, b ? incl[b - 1] : 0.0, s, e});
-        s = e;
-      }
-      std::vector<uint8_t> blob(shots * 8 + jobs.size() * sizeof(BlockJob));
-      std::memcpy(blob.data(), us.data(), shots * 8);
-      if (!jobs.empty()) std::memcpy(blob.data() + shots * 8, jobs.data(), jobs.size() * sizeof(BlockJob));
-      uint8_t* d = upload(st, st->aux, blob);
-      st->aux2.reserve(shots * 8);
-      hip_check(hipMemsetAsync(st->aux2.p, 0xff, shots * 8, st->stream), "memset");
-      ProfScope ps_(st, "sample_blocks", (double)jobs.size() * (double)(1ull << bb) * st->bytes_per_amp());
-      hip_check(launch_sample_blocks(st->prec, st->amp, st->dim, bb, (const BlockJob*)(d + shots * 8), jobs.size(),
-                                     (const double*)d, (uint64_t*)st->aux2.p, st->stream),
-                "sample blocks");
+    } else {
+      sample_exact_parallel(st, us, res);
     }
-    hip_check(hipMemcpyAsync(res.data(), st->aux2.p, shots * 8, hipMemcpyDeviceToHost, st->stream), "D2H");
-    hip_check(hipStreamSynchronize(st->stream), "sync");
     for (uint64_t s = 0; s < shots; ++s) idx_out[perm[s]] = res[s];
   });
 }

@@ -67,9 +67,12 @@ struct TileArgs {
   uint32_t pad;
 };
 
-struct BlockJob {
+// A block of the exact block-parallel sampler whose fold stays in one binade: its draws are
+// u[first .. last), its fold starts at m_start units of `unit` = 2^(e-52), scale = 1 / unit.
+struct ExactJob {
   uint64_t block;
-  double start;
+  double m_start;
+  double unit, scale;
   uint64_t first, last;
 };
 
@@ -96,8 +99,12 @@ hipError_t launch_tile(int prec, int K, int RB, bool prefetch, void* psi, const 
 hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const double* u, uint64_t shots,
                               uint64_t* out, hipStream_t s);
 hipError_t launch_block_mass(int prec, const void* psi, uint64_t dim, int bb, double* mass, hipStream_t s);
-hipError_t launch_sample_blocks(int prec, const void* psi, uint64_t dim, int bb, const BlockJob* jobs,
-                                uint64_t njobs, const double* u, uint64_t* out, hipStream_t s);
+// Exact sums of rint(p_i * 2^(52-e)) per block for e = ebase[b] + 0..2, and halfway flags.
+hipError_t launch_block_exact_sums(int prec, const void* psi, uint64_t dim, int bb, const int32_t* ebase,
+                                   double* sums, uint32_t* flags, hipStream_t s);
+// Resolves each job's draws with its block's exact fold values (bb >= 12).
+hipError_t launch_sample_exact_blocks(int prec, const void* psi, uint64_t dim, int bb, const ExactJob* jobs,
+                                      uint64_t njobs, const double* u, uint64_t* out, hipStream_t s);
 hipError_t launch_norm_partial(int prec, const void* psi, uint64_t dim, double* part, int nblocks,
                                hipStream_t s);
 hipError_t launch_gen_partial(int prec, const void* lam, const void* psi, uint64_t dim, int t, int gen,

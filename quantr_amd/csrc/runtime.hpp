@@ -166,6 +166,9 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
 void run_exact(qr_state* st, const std::vector<ExactGate>& gates, qr_plan_stats* stats);
 void do_apply(qr_state* st, const qr_program* prog);
 double sum_partials_host(const std::vector<double>& v);
+// measure_all's sampling (sampler.cpp): draws `us` in ascending order; res[s] = the first index
+// whose left-fold cumulative probability exceeds us[s], or UINT64_MAX (None).
+void sample_exact_parallel(qr_state* st, const std::vector<double>& us, std::vector<uint64_t>& res);
 double norm_sqr(qr_state* st);
 std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o);
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms);

@@ -161,6 +161,19 @@ def inverse(gates, customs=None):
     return out, out_custom
 
 
+def batch_instances(count, seed=512):
+    """The batched-serving workload: `count` HEA instances with n ~ U{10..24} qubits and
+    L ~ U{2..12} layers, each with its own angles. Returns [(n, L, gates, thetas), ...]."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(count):
+        n = 10 + rng.below(15)
+        L = 2 + rng.below(11)
+        gates, thetas = hea(n, L, seed=rng.next_u64())
+        out.append((n, L, gates, thetas))
+    return out
+
+
 def uniforms(count, seed):
     rng = SplitMix64(seed)
     return [rng.uniform() for _ in range(count)]

@@ -132,10 +132,11 @@ def test_qft10_state_and_bins():
             assert np.max(np.abs(got - want)) < TOL_FUSED
             bins = st.sample(u)
             st.close()
-            if fused == 0:
-                assert np.array_equal(bins, want_bins)
-            else:  # fused state differs in the last bits: bins equal unless a draw sits on a boundary
-                assert np.mean(bins == want_bins) > 0.99
+            # the sampler reproduces the left fold exactly on the state it holds ...
+            assert np.array_equal(bins, H.oracle_measure(10, got, u))
+            # ... and the fused state (within 1e-12 of the reference's) gives the reference's
+            # bins for these draws: none lies within 1e-12 of a cumulative-probability boundary
+            assert np.array_equal(bins, want_bins)
 
 
 def test_sampler_exact_fold_bitwise():
@@ -155,7 +156,7 @@ def test_sampler_exact_fold_bitwise():
         assert np.array_equal(got, want)
         st.set_options(exact_sample=0)
         got2 = st.sample(u)
-        assert np.mean(got2 == want) > 0.999
+        assert np.array_equal(got2, want)
         st.close()
 
 
