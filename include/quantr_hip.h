@@ -140,9 +140,11 @@ typedef struct qr_options {
                             0: one sweep per gate in the reference's exact arithmetic order */
   uint32_t tile_bits;    /* fused tile width k (0 = auto)                              */
   uint32_t low_bits;     /* contiguous low bits per tile segment c (0 = auto)          */
-  uint32_t exact_sample; /* 1: sampler reproduces the reference's sequential left fold
-                            bit for bit; 0: parallel block scan (block sums reassociate
-                            the fold); 2 (default): exact when n <= 26, else parallel   */
+  uint32_t exact_sample; /* sampler; every mode reproduces the reference's sequential left
+                            fold bit for bit. 0 and 2 (default): block-parallel (exact
+                            integer sums inside each binade of the fold, sequential folds
+                            only where it crosses a binade); 1: one lane folds the whole
+                            state in order (a checker; slow for large states)           */
   uint32_t tile_variant; /* 0: auto; 1: one tile in registers per workgroup;
                             2: next tile prefetched into registers (double buffer)       */
   uint32_t reg_bits;     /* amplitudes per thread = 2^reg_bits (0 = auto, 3, 4 or 5)    */
@@ -212,7 +214,6 @@ qr_status qr_state_norm_sqr(qr_state* state, double* out);
 
 /* <psi|O|psi> in f64 (no reference counterpart). */
 qr_status qr_expectation(qr_state* state, const qr_observable* observable, double* out);
-
 /* ---- adjoint differentiation (no reference counterpart) -------------------------------- */
 
 /* A forward/backward pair with the contract of an autograd Function:
