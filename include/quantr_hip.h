@@ -214,6 +214,17 @@ qr_status qr_state_norm_sqr(qr_state* state, double* out);
 
 /* <psi|O|psi> in f64 (no reference counterpart). */
 qr_status qr_expectation(qr_state* state, const qr_observable* observable, double* out);
+
+/* qr_expectation with no allocation of any kind (no hipMalloc, hipFree or allocating library
+ * call): no counterpart in the reference; for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). The kernel's working
+ * set (observable tables and partial sums) lives in a fixed 1 MiB device array per GPU, held
+ * until the process ends. At most 16384 terms (more is QR_ERR_ARG). The result is bitwise equal
+ * to qr_expectation's. Every call on a GPU shares that GPU's array, which keeps the last call's
+ * contents (nothing in it has to start at zero); calls would race on it if they overlapped, so
+ * the library serialises them with a host mutex and each returns only after its kernel ends. */
+qr_status qr_expectation_noalloc(qr_state* state, const qr_observable* observable, double* out);
+
 /* ---- adjoint differentiation (no reference counterpart) -------------------------------- */
 
 /* A forward/backward pair with the contract of an autograd Function:

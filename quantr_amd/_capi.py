@@ -84,6 +84,7 @@ _SIGNATURES = [
     ("qr_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
     ("qr_state_norm_sqr", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qr_expectation", C.c_int, [C.c_void_p, C.POINTER(QrObservable), C.POINTER(C.c_double)]),
+    ("qr_expectation_noalloc", C.c_int, [C.c_void_p, C.POINTER(QrObservable), C.POINTER(C.c_double)]),
     ("qr_grad_create", C.c_int, [C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
     ("qr_grad_destroy", C.c_int, [C.c_void_p]),
     ("qr_grad_forward", C.c_int, [C.c_void_p, C.POINTER(QrProgram), C.POINTER(QrObservable),

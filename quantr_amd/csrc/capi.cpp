@@ -254,6 +254,13 @@ qr_status qr_expectation(qr_state* st, const qr_observable* observable, double* 
   });
 }
 
+qr_status qr_expectation_noalloc(qr_state* st, const qr_observable* observable, double* out) {
+  return guard([&] {
+    if (!st || !out) throw Error{QR_ERR_ARG, "NULL argument"};
+    *out = expectation_noalloc(st, observable);
+  });
+}
+
 // ---- adjoint differentiation ------------------------------------------------------------
 
 qr_status qr_grad_create(uint32_t num_qubits, qr_precision precision, qr_grad** out) {

@@ -419,9 +419,10 @@ void do_apply(qr_state* st, const qr_program* prog) {
   run_fast(st, ops, true, &st->last);
 }
 
-double sum_partials_host(const std::vector<double>& v) {
+double sum_partials_host(const double* v, size_t count) {
   double s = 0.0, c = 0.0;
-  for (double x : v) {  // Neumaier
+  for (size_t i = 0; i < count; ++i) {  // Neumaier
+    const double x = v[i];
     const double t = s + x;
     if (std::fabs(s) >= std::fabs(x)) c += (s - t) + x;
     else c += (x - t) + s;
@@ -429,6 +430,8 @@ double sum_partials_host(const std::vector<double>& v) {
   }
   return s + c;
 }
+
+double sum_partials_host(const std::vector<double>& v) { return sum_partials_host(v.data(), v.size()); }
 
 double norm_sqr(qr_state* st) {
   const int nb = 1024;
@@ -440,32 +443,45 @@ double norm_sqr(qr_state* st) {
   return sum_partials_host(h);
 }
 
+PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t) {
+  const qr_pauli_term& pt = o->terms[t];
+  if (pt.nfactors && (!o->wires || !o->paulis || pt.offset + pt.nfactors > o->nfactors))
+    throw Error{QR_ERR_ARG, "observable term " + std::to_string(t) + " is out of range"};
+  PauliMasks tm{pt.coeff, 0, 0, 0};
+  uint64_t seen = 0;
+  for (uint32_t f = 0; f < pt.nfactors; ++f) {
+    const uint32_t w = o->wires[pt.offset + f];
+    if (w >= n) throw Error{QR_ERR_ARG, "observable wire " + std::to_string(w) + " is out of range"};
+    const uint64_t b = 1ull << (n - 1 - w);
+    if (seen & b) throw Error{QR_ERR_ARG, "observable term repeats wire " + std::to_string(w)};
+    seen |= b;
+    switch (o->paulis[pt.offset + f]) {
+      case 'X': case 1: tm.xm |= b; break;
+      case 'Y': case 2: tm.xm |= b; tm.zm |= b; tm.ny++; break;
+      case 'Z': case 3: tm.zm |= b; break;
+      case 'I': case 0: break;
+      default: throw Error{QR_ERR_ARG, "observable pauli must be X, Y or Z"};
+    }
+  }
+  return tm;
+}
+
 std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o) {
   if (!o) throw Error{QR_ERR_ARG, "observable is NULL"};
+  if (o->nterms && !o->terms) throw Error{QR_ERR_ARG, "observable terms are NULL"};
   std::vector<PauliMasks> out;
-  for (uint64_t t = 0; t < o->nterms; ++t) {
-    const qr_pauli_term& pt = o->terms[t];
-    if (pt.nfactors && (!o->wires || !o->paulis || pt.offset + pt.nfactors > o->nfactors))
-      throw Error{QR_ERR_ARG, "observable term " + std::to_string(t) + " is out of range"};
-    PauliMasks tm{pt.coeff, 0, 0, 0};
-    uint64_t seen = 0;
-    for (uint32_t f = 0; f < pt.nfactors; ++f) {
-      const uint32_t w = o->wires[pt.offset + f];
-      if (w >= n) throw Error{QR_ERR_ARG, "observable wire " + std::to_string(w) + " is out of range"};
-      const uint64_t b = 1ull << (n - 1 - w);
-      if (seen & b) throw Error{QR_ERR_ARG, "observable term repeats wire " + std::to_string(w)};
-      seen |= b;
-      switch (o->paulis[pt.offset + f]) {
-        case 'X': case 1: tm.xm |= b; break;
-        case 'Y': case 2: tm.xm |= b; tm.zm |= b; tm.ny++; break;
-        case 'Z': case 3: tm.zm |= b; break;
-        case 'I': case 0: break;
-        default: throw Error{QR_ERR_ARG, "observable pauli must be X, Y or Z"};
-      }
-    }
-    out.push_back(tm);
-  }
+  for (uint64_t t = 0; t < o->nterms; ++t) out.push_back(parse_term(n, o, t));
   return out;
+}
+
+PauliZTerm z_term(const PauliMasks& tm) {
+  double cr = tm.coeff, ci = 0.0;
+  for (int q = 0; q < (tm.ny & 3); ++q) {  // i^ny folded into the coefficient
+    const double t = cr;
+    cr = -ci;
+    ci = t;
+  }
+  return PauliZTerm{tm.zm, cr, ci};
 }
 
 // <psi|O|psi> in one pass over the state (terms grouped by X mask), partials summed on the host.
@@ -495,15 +511,7 @@ void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>&
                   std::vector<PauliZTerm>& flat) {
   // terms grouped by X mask; Z masks and i^ny folded into per-term complex coefficients
   std::map<uint64_t, std::vector<PauliZTerm>> by_x;
-  for (const auto& tm : terms) {
-    double cr = tm.coeff, ci = 0.0;
-    for (int q = 0; q < (tm.ny & 3); ++q) {
-      const double t = cr;
-      cr = -ci;
-      ci = t;
-    }
-    by_x[tm.xm].push_back(PauliZTerm{tm.zm, cr, ci});
-  }
+  for (const auto& tm : terms) by_x[tm.xm].push_back(z_term(tm));
   groups.clear();
   flat.clear();
   for (const auto& g : by_x) {

@@ -166,12 +166,20 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
 void run_exact(qr_state* st, const std::vector<ExactGate>& gates, qr_plan_stats* stats);
 void do_apply(qr_state* st, const qr_program* prog);
 double sum_partials_host(const std::vector<double>& v);
+// Compensated (Neumaier) sum of count partials, in index order.
+double sum_partials_host(const double* v, size_t count);
 // measure_all's sampling (sampler.cpp): draws `us` in ascending order; res[s] = the first index
 // whose left-fold cumulative probability exceeds us[s], or UINT64_MAX (None).
 void sample_exact_parallel(qr_state* st, const std::vector<double>& us, std::vector<uint64_t>& res);
 double norm_sqr(qr_state* st);
 std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o);
+// Term t of an observable as masks (checked: wires in range, no repeated wire, X/Y/Z only).
+PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t);
+// A term as the device's Z-mask term: the coefficient times i^ny.
+PauliZTerm z_term(const PauliMasks& tm);
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms);
+// The same value through fixed device and host arrays: no allocation (noalloc.cpp).
+double expectation_noalloc(qr_state* st, const qr_observable* o);
 // The observable as device tables: terms grouped by X mask (i^ny folded into coefficients).
 void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
                   std::vector<PauliZTerm>& flat);
