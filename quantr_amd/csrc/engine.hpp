@@ -115,7 +115,8 @@ std::vector<FastOp> adjoint_sweep_ops(const std::vector<FastOp>& ops);
 struct TileOpHost {
   uint32_t code = 0;
   uint32_t a = 0;        // U1: register slot; D1: global bit; REMAP/SWAPSLOTS: new map index
-  uint32_t b = 0;        // U1: form; D1: register slot holding the bit, or 0xFF
+  uint32_t b = 0;        // U1: form; D1: register slot holding the bit, or 0xFF;
+                         // REMAP with flags bit 0 (split): the tile position splitting rounds
   uint32_t flags = 0;    // D1: bit 0 = only the |1> half changes; bits 4-7 = factor form
   uint32_t ns = 0;       // register slots that must be 1 (controls held in registers)
   uint64_t lane_cmask = 0;  // control bits held by thread slots or outside the tile
@@ -146,14 +147,36 @@ struct Stage {
   int standalone = -1;         // index into the op list for standalone stages
   double bytes = 0.0;          // algorithmic bytes of this sweep (2 * 2^n * B * f_touch)
   int ngates = 0;
+  // physical index bit of each tile position when the tile is loaded and when it is stored
+  // (they differ when the stage moves bits into the physical low bits); maps address the
+  // load layout except map store_map (the last one), which addresses the store layout
+  std::vector<int> pload, pstore;
+  int store_map = -1;
+  int phys_bit(int map_index, int pos) const {
+    return map_index == store_map ? pstore[pos] : pload[pos];
+  }
 };
 
 struct PlanParams {
   int tile_bits = 12;
   int reg_bits = 4;
   int low_bits = 4;
-  int max_ops = 0;  // ops per tile stage at most (0 = no limit)
+  int max_ops = 0;    // ops per tile stage at most (0 = no limit)
+  int lds_split = 0;  // remaps exchange the tile through LDS in two halves (half the LDS)
+  int dyn_low = 0;    // moving layout: stages choose which bits the physical low bits hold
 };
+
+// A remap done in two halves ("split", TileOpHost::flags bit 0) moves the amplitudes whose
+// tile position `b` is 0 in the first round and those where it is 1 in the second; position b
+// carries the top LDS address bit in the pattern table, which rounds drop. In each round a
+// thread writes the registers it holds of that half (all of them, or none, when b is a thread
+// slot) and then reads the ones it holds of that half in the new map. A first-round read
+// whose register is still to be written in the second round goes to a temporary.
+//   old_slot / new_slot: the slot holding position b in the old / new map; nts: thread slots.
+inline bool split_read_needs_tmp(int old_slot, int new_slot, int nts, int m) {
+  if (old_slot < nts) return !(new_slot == old_slot);       // written by all or no registers later
+  return ((m >> (old_slot - nts)) & 1) != 0;                 // register m is written in round 1
+}
 
 // Stage op limit when one tile covers the whole state: such a stage could otherwise absorb the
 // entire program into one compiled kernel (long compiles, large code); a split costs one launch

@@ -66,16 +66,20 @@ std::string hex64(uint64_t v) {
 // (TC_GRAD) on them; each tap stores per-wavefront partials into its parameter's row.
 std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool dual) {
   const int K = s.k, NT = K - RB, NR = 1 << RB, TPB = 1 << NT;
-  uint64_t tile_mask = 0;
-  for (int b : s.tbits) tile_mask |= 1ull << b;
+  uint64_t tile_mask = 0;  // physical bits of the tile
+  for (int b : s.pload) tile_mask |= 1ull << b;
   const char* R = prec == 0 ? "double" : "float";
   const char* C = prec == 0 ? "double2" : "float2";
   const int amp_bytes = prec == 0 ? 16 : 8;
   // both states through LDS at once when that fits in 64 KB, else one after the other
   const bool lds2 = dual && (2 * (1 << K) * amp_bytes <= 65536);
+  // remaps in two halves (forward stages only): the LDS buffer holds half the tile
+  bool split = false;
+  if (!dual)
+    for (const TileOpHost& op : s.ops) split = split || (op.code == TC_REMAP && (op.flags & 1));
   std::vector<std::string> sets = dual ? std::vector<std::string>{"r", "l"} : std::vector<std::string>{"r"};
   std::vector<std::string> arrays = dual ? std::vector<std::string>{"psi", "lam"} : std::vector<std::string>{"psi"};
-  auto gbit = [&](int mi, int slot) { return s.tbits[s.maps[(size_t)mi * K + slot]]; };
+  auto gbit = [&](int mi, int slot) { return s.phys_bit(mi, s.maps[(size_t)mi * K + slot]); };
   auto thread_part = [&](int mi) {
     std::ostringstream os;
     os << "base";
@@ -89,8 +93,19 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
     return x;
   };
   std::ostringstream o;
-  o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")"
-    << (persistent ? " __attribute__((amdgpu_waves_per_eu(2, 2)))" : "") << " void "
+  // occupancy target of the stage kernels: QR_STAGE_WAVES_PER_EU (1-8) asks the compiler to
+  // fit that many wavefronts per SIMD (fewer registers per thread); unset leaves it free
+  int wpe = persistent ? 2 : 0;
+  // a 256-thread stage exchanging through half the LDS fits three workgroups per CU when it
+  // uses at most 168 registers per lane: ask the compiler for three wavefronts per SIMD
+  if (split && TPB == 256) wpe = 3;
+  if (const char* e = std::getenv("QR_STAGE_WAVES_PER_EU")) {
+    const int v = std::atoi(e);
+    if (v >= 1 && v <= 8) wpe = v;
+  }
+  o << "extern \"C\" __global__ __launch_bounds__(" << TPB << ")";
+  if (wpe) o << " __attribute__((amdgpu_waves_per_eu(" << wpe << ", " << wpe << ")))";
+  o << " void "
     << (dual ? "qr_dual_stage(" : "qr_stage(") << C
     << "* __restrict__ psi, ";
   if (dual) o << C << "* __restrict__ lam, ";
@@ -98,7 +113,7 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   if (dual) o << ", double* __restrict__ part, uint64_t nslots";
   o << ") {\n";
   o << "  using R = " << R << ";\n";
-  o << "  __shared__ " << C << " lds[" << (lds2 ? 2 : 1) * (1 << K) << "];\n";
+  o << "  __shared__ " << C << " lds[" << (split ? (1 << (K - 1)) : (lds2 ? 2 : 1) * (1 << K)) << "];\n";
   o << "  const uint32_t tid = threadIdx.x;\n";
   o << "  uint64_t tile = blockIdx.x;\n";
   o << "  if (tile >= ntiles) return;\n";
@@ -224,6 +239,50 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             if ((m >> j) & 1) x ^= Q[NT + j];
           return x;
         };
+        if (split && (op.flags & 1)) {
+          // two rounds through half the LDS, split by tile position op.b
+          const int h = (int)op.b, nmi = (int)op.a;
+          int osl = -1, nsl = -1;
+          for (int sl = 0; sl < K; ++sl) {
+            if (s.maps[(size_t)mi * K + sl] == h) osl = sl;
+            if (s.maps[(size_t)nmi * K + sl] == h) nsl = sl;
+          }
+          const uint32_t hm = (1u << (K - 1)) - 1;
+          o << "  { const uint32_t w = (" << lds_thread(P) << ") & " << hm << "u;\n";
+          o << "    const uint32_t q = (" << lds_thread(P + K) << ") & " << hm << "u;\n";
+          auto in_round = [&](int slot, int m, int rho) {  // register m of this half (register slot)
+            return ((m >> (slot - NT)) & 1) == rho;
+          };
+          std::vector<int> tmp;
+          for (int m = 0; m < NR; ++m)
+            if ((nsl < NT || in_round(nsl, m, 0)) && split_read_needs_tmp(osl, nsl, NT, m)) tmp.push_back(m);
+          for (int m : tmp) o << "    " << C << " t" << m << ";\n";
+          for (int rho = 0; rho < 2; ++rho) {
+            o << "    __syncthreads();\n";
+            if (osl < NT) o << "    if (((tid >> " << osl << ") & 1u) == " << rho << "u) {\n";
+            for (int m = 0; m < NR; ++m)
+              if (osl < NT || in_round(osl, m, rho))
+                o << "    lds[w ^ " << (lds_reg(P, m) & hm) << "u] = r.v[" << m << "];\n";
+            if (osl < NT) o << "    }\n";
+            o << "    __syncthreads();\n";
+            if (nsl < NT) o << "    if (((tid >> " << nsl << ") & 1u) == " << rho << "u) {\n";
+            for (int m = 0; m < NR; ++m) {
+              if (!(nsl < NT || in_round(nsl, m, rho))) continue;
+              const bool t = rho == 0 && std::find(tmp.begin(), tmp.end(), m) != tmp.end();
+              o << "    " << (t ? "t" + std::to_string(m) : "r.v[" + std::to_string(m) + "]") << " = lds[q ^ "
+                << (lds_reg(P + K, m) & hm) << "u];\n";
+            }
+            if (nsl < NT) o << "    }\n";
+          }
+          if (!tmp.empty()) {
+            if (nsl < NT) o << "    if (((tid >> " << nsl << ") & 1u) == 0u) {\n";
+            for (int m : tmp) o << "    r.v[" << m << "] = t" << m << ";\n";
+            if (nsl < NT) o << "    }\n";
+          }
+          mi = nmi;
+          o << "    g0 = " << thread_part(mi) << "; }\n";
+          break;
+        }
         o << "  { const uint32_t w = " << lds_thread(P) << ";\n";
         o << "    const uint32_t q = " << lds_thread(P + K) << ";\n";
         if (lds2 || !dual) {

@@ -284,6 +284,57 @@ std::vector<uint16_t> choose_swizzle(int k, int nts, const std::vector<int>& old
   return S;
 }
 
+// Pattern table of a remap done in two halves split by tile position h: the other positions
+// get a conflict-free table over k-1 bits (choose_swizzle on the maps without h) and h gets
+// the top bit, so the table is still a bijection onto 2^k slots (a kernel with the whole
+// tile in LDS can run the remap in one round) while each round of the split exchange uses
+// the low 2^(k-1) slots.
+std::vector<uint16_t> choose_swizzle_split(int k, int nts, const std::vector<int>& oldm, const std::vector<int>& newm,
+                                           int h) {
+  auto reduce = [&](const std::vector<int>& m) {
+    std::vector<int> r;
+    for (int p : m)
+      if (p != h) r.push_back(p > h ? p - 1 : p);
+    return r;
+  };
+  const std::vector<uint16_t> t = choose_swizzle(k - 1, nts, reduce(oldm), reduce(newm));
+  std::vector<uint16_t> S(k);
+  for (int p = 0; p < k; ++p)
+    if (p != h) S[p] = t[p > h ? p - 1 : p];
+  S[h] = (uint16_t)(1u << (k - 1));
+  return S;
+}
+
+// The tile position that splits a two-round remap from map `om` to map `nm`, best first:
+// a register position in both maps at the same register slot (no temporaries), a register
+// position in both maps, the same wavefront-index thread slot in both maps (each wavefront
+// takes part in one round), a register position on one side and a wavefront-index slot on
+// the other, then any position off lane slots 0-2 (their patterns carry the LDS swizzle).
+int choose_split_pos(int k, int nts, const std::vector<int>& om, const std::vector<int>& nm) {
+  std::vector<int> os(k), ns(k);
+  for (int sl = 0; sl < k; ++sl) {
+    os[om[sl]] = sl;
+    ns[nm[sl]] = sl;
+  }
+  constexpr int kWaveSlot = 6;  // thread slots from 6 on index whole 64-lane wavefronts
+  int best = -1, best_score = -1;
+  for (int p = 0; p < k; ++p) {
+    const bool oreg = os[p] >= nts, nreg = ns[p] >= nts;
+    int score;
+    if (oreg && nreg) score = os[p] == ns[p] ? 6 : 5;
+    else if (!oreg && !nreg && os[p] == ns[p] && os[p] >= kWaveSlot) score = 4;
+    else if ((oreg && ns[p] >= kWaveSlot) || (nreg && os[p] >= kWaveSlot)) score = 3;
+    else if (os[p] >= kWaveSlot && ns[p] >= kWaveSlot) score = 2;
+    else if (os[p] >= 3 && ns[p] >= 3) score = 1;
+    else continue;
+    if (score > best_score) {
+      best_score = score;
+      best = p;
+    }
+  }
+  return best;
+}
+
 }  // namespace
 
 void plan_key_append(const FastOp& op, std::vector<uint64_t>& key) {
@@ -325,15 +376,35 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
   const int r = std::min(pp.reg_bits, k);
   const int nts = k - r;  // thread slots
   const int c = std::min(pp.low_bits, nts);
+  const bool split = pp.lds_split != 0 && nts >= 4;
   const double sweep_bytes = 2.0 * (double)(1ull << n) * bytes_per_amp;
-  uint64_t low_mask = (c > 0) ? ((1ull << c) - 1) : 0;
+  // Layout: logical index bit l (wire n-1-l) is stored at physical bit phys[l]. Without a
+  // moving layout it is the identity. With one (dyn), each stage may store some of its tile
+  // bits at the c physical low bits, so that the next stage's contiguous segments hold bits it
+  // uses. The layout stays a product of at most c transpositions between a low physical bit j
+  // and the home of the logical bit it holds, and is the identity again before a standalone op
+  // and after the last stage (by an op-free restoring sweep if no stage can do it).
+  const bool dyn = pp.dyn_low != 0 && !dual && c > 0 && k < (int)n;
+  std::vector<int> phys(n), logi(n);
+  for (uint32_t b = 0; b < n; ++b) phys[b] = logi[b] = (int)b;
+  auto identity_layout = [&]() {
+    for (uint32_t b = 0; b < n; ++b)
+      if (phys[b] != (int)b) return false;
+    return true;
+  };
+  std::vector<int> all_bits(n);
+  for (uint32_t b = 0; b < n; ++b) all_bits[b] = (int)b;
 
   std::vector<int> remaining(N);
   for (int i = 0; i < N; ++i) remaining[i] = i;
 
-  while (!remaining.empty()) {
-    const int front = remaining[0];
-    if (!eligible[front]) {
+  while (!remaining.empty() || (dyn && !identity_layout())) {
+    // logical bits held by the physical low bits (the contiguous segment of every tile)
+    uint64_t low_mask = 0;
+    for (int j = 0; j < c; ++j) low_mask |= 1ull << logi[j];
+    const bool restore_only = remaining.empty() || (dyn && !identity_layout() && !eligible[remaining[0]]);
+    const int front = restore_only ? -1 : remaining[0];
+    if (front >= 0 && !eligible[front]) {
       Stage s;
       s.tile = false;
       s.standalone = front;
@@ -344,33 +415,218 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       continue;
     }
     // ---- choose the tile bits B ------------------------------------------------------
-    std::vector<int> all_bits(n);
-    for (uint32_t b = 0; b < n; ++b) all_bits[b] = (int)b;
-    // the front op's targets, then the low bits (coalescing), then greedy
-    uint64_t B = mk[front].t;
-    for (int b = 0; b < c && __builtin_popcountll(B) < k; ++b) B |= 1ull << b;
+    // the front op's targets, then the logical bits at the physical low bits (coalescing),
+    // then greedy; an op-free restoring sweep takes every displaced bit and its partner
+    uint64_t B = front >= 0 ? mk[front].t : 0;
+    B |= low_mask;
+    if (restore_only)
+      for (uint32_t b = 0; b < n; ++b)
+        if (phys[b] != (int)b || (int)b < c) B |= 1ull << b;
     int have = __builtin_popcountll(B);
-    if (have < k) B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible);
+    if (have < k && !restore_only) B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible);
     // fill up if greedy ran out of useful bits (keeps k fixed for the kernel variant)
     for (uint32_t b = 0; b < n && __builtin_popcountll(B) < k; ++b) B |= 1ull << b;
+    if (dyn && !restore_only && !identity_layout()) {
+      // when every op left fits in one tile that also holds every displaced bit, that tile is
+      // the last stage and restores the layout (no op-free restoring sweep after it)
+      uint64_t Bl = mk[front].t | low_mask;
+      for (uint32_t b = 0; b < n; ++b)
+        if (phys[b] != (int)b) Bl |= 1ull << b;
+      const bool standalone_later = std::any_of(remaining.begin(), remaining.end(), [&](int i) { return !eligible[i]; });
+      if (!standalone_later && __builtin_popcountll(Bl) <= k) {
+        const int hv = __builtin_popcountll(Bl);
+        if (hv < k) Bl = greedy_bits(Bl, k - hv, all_bits, remaining, mk, eligible);
+        for (uint32_t b = 0; b < n && __builtin_popcountll(Bl) < k; ++b) Bl |= 1ull << b;
+        if (count_absorb(remaining, mk, eligible, Bl) == (int)remaining.size() &&
+            (pp.max_ops <= 0 || (int)remaining.size() <= pp.max_ops))
+          B = Bl;
+      }
+    }
 
     std::vector<int> taken, left;
-    absorb(remaining, mk, eligible, B, &taken, &left);
-    if (pp.max_ops > 0 && (int)taken.size() > pp.max_ops) {
-      // defer the tail; the remaining list stays in program order
-      left.insert(left.end(), taken.begin() + pp.max_ops, taken.end());
-      taken.resize(pp.max_ops);
-      std::sort(left.begin(), left.end());
+    if (!restore_only) {
+      absorb(remaining, mk, eligible, B, &taken, &left);
+      if (pp.max_ops > 0 && (int)taken.size() > pp.max_ops) {
+        // defer the tail; the remaining list stays in program order
+        left.insert(left.end(), taken.begin() + pp.max_ops, taken.end());
+        taken.resize(pp.max_ops);
+        std::sort(left.begin(), left.end());
+      }
+    } else {
+      left = remaining;
     }
+    if (taken.empty() && !restore_only) throw Error{QR_ERR_UNSUPPORTED, "internal planner error: a stage takes no op"};
+
+    // ---- the layout this stage stores (moving layout only) ----------------------------
+    std::vector<int> nphys = phys, nlogi = logi;
+    if (dyn) {
+      // the bits the next stage would choose if c of them must come from this tile: its front
+      // op's targets, its best free picks, then (if the picks hold fewer than c of this tile's
+      // bits) the best of this tile's bits
+      uint64_t Bn = 0;
+      const bool next_tile = !left.empty() && eligible[left[0]];
+      if (next_tile) {
+        Bn = mk[left[0]].t;
+        const int need = std::max(0, c - __builtin_popcountll(Bn & B));
+        const int hv = __builtin_popcountll(Bn);
+        if (hv + need < k) Bn = greedy_bits(Bn, k - hv - need, all_bits, left, mk, eligible);
+        const int have_mine = __builtin_popcountll(Bn & B);
+        if (have_mine < c) {
+          std::vector<int> mine;
+          for (uint32_t b = 0; b < n; ++b)
+            if ((B & (1ull << b)) && !(Bn & (1ull << b))) mine.push_back((int)b);
+          Bn = greedy_bits(Bn, std::min(c - have_mine, k - __builtin_popcountll(Bn)), mine, left, mk, eligible);
+        }
+        const int hv2 = __builtin_popcountll(Bn);
+        if (hv2 < k) Bn = greedy_bits(Bn, k - hv2, all_bits, left, mk, eligible);
+      }
+      // first use of each bit among the ops left (sooner first)
+      std::vector<int> first_use(n, 1 << 30);
+      for (size_t q = 0; q < left.size(); ++q) {
+        uint64_t m = mk[left[q]].t | mk[left[q]].r;
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1;
+          if (first_use[b] > (int)q) first_use[b] = (int)q;
+        }
+      }
+      // A candidate layout: for each low slot j, keep its bit when `keep` says so, else lower
+      // the soonest-used bit of this tile that `want` contains. The bit leaving slot j goes
+      // home when its home is in the tile, else to the place of the bit that replaces it. A
+      // move is kept only while the displaced bits and the low bits fit in one tile, so one
+      // op-free sweep can always restore the identity. Before a standalone op and at the end
+      // every displaced bit goes home when all of them are in this tile.
+      auto displaced_fit = [&](const std::vector<int>& cp) {
+        uint64_t m = 0;
+        for (uint32_t b = 0; b < n; ++b)
+          if (cp[b] != (int)b) m |= 1ull << b;
+        for (int j = 0; j < c; ++j) m |= 1ull << j;  // the low bits' own logical bits
+        return __builtin_popcountll(m) <= k;
+      };
+      // when a tile with the identity's low bits takes every op left, the next stage is the last:
+      // restore the identity now, so that it needs no op-free restoring sweep after it
+      bool final_next = false;
+      if (next_tile) {
+        uint64_t Bi = mk[left[0]].t;
+        for (int j = 0; j < c; ++j) Bi |= 1ull << j;
+        if (__builtin_popcountll(Bi) <= k) {
+          const int hv = __builtin_popcountll(Bi);
+          if (hv < k) Bi = greedy_bits(Bi, k - hv, all_bits, left, mk, eligible);
+          final_next = count_absorb(left, mk, eligible, Bi) == (int)left.size() &&
+                       (pp.max_ops <= 0 || (int)left.size() <= pp.max_ops);
+        }
+      }
+      auto candidate = [&](uint64_t want, bool keep_useful, std::vector<int>& cp, std::vector<int>& cl) {
+        cp = phys;
+        cl = logi;
+        if (!next_tile || final_next) {
+          // every displaced bit of this tile whose home is in the tile goes home (the bit there
+          // takes its place); each move fixes one bit more, so this ends
+          bool moved = true;
+          while (moved) {
+            moved = false;
+            for (uint32_t b = 0; b < n; ++b) {
+              if (cp[b] == (int)b || !(B & (1ull << b))) continue;
+              const int z = cl[b];  // the bit at b's home
+              if (!(B & (1ull << z))) continue;
+              const int pb = cp[b];
+              cp[b] = (int)b;
+              cp[z] = pb;
+              cl[b] = (int)b;
+              cl[pb] = z;
+              moved = true;
+            }
+          }
+          return;
+        }
+        for (int j = 0; j < c; ++j) {
+          const int x = cl[j];
+          if (keep_useful && (want & (1ull << x))) continue;
+          int y = -1, best = 1 << 30;
+          for (uint32_t b = 0; b < n; ++b)
+            if ((B & (1ull << b)) && (want & (1ull << b)) && cp[b] >= c && first_use[b] < best) {
+              best = first_use[b];
+              y = (int)b;
+            }
+          if (y < 0 || y == x) continue;
+          std::vector<int> tp = cp;
+          const int py = tp[y];
+          tp[y] = j;
+          const bool home_in_tile = (B & (1ull << cl[x])) != 0;  // the bit now at x's home
+          if (home_in_tile && x != cl[x] && cl[x] != y) {
+            const int z = cl[x];  // x home, z to y's place
+            tp[x] = x;
+            tp[z] = py;
+          } else {
+            tp[x] = py;  // x to y's place (x's home when y sat there)
+          }
+          if (!displaced_fit(tp)) continue;
+          cp = tp;
+          for (uint32_t b = 0; b < n; ++b) cl[cp[b]] = (int)b;
+        }
+      };
+      // ops the next stage absorbs when its low bits are those of layout cl
+      auto next_absorbs = [&](const std::vector<int>& cl) {
+        uint64_t Bx = mk[left[0]].t;
+        for (int j = 0; j < c; ++j) Bx |= 1ull << cl[j];
+        const int hv = __builtin_popcountll(Bx);
+        if (hv < k) Bx = greedy_bits(Bx, k - hv, all_bits, left, mk, eligible);
+        return count_absorb(left, mk, eligible, Bx);
+      };
+      candidate(Bn, true, nphys, nlogi);
+      if (next_tile && !final_next) {
+        // other candidates: the soonest-used tile bits, keeping useful low bits or not, and the
+        // unconstrained pick without keeping; the one whose next stage absorbs most wins
+        uint64_t soon = 0;
+        for (uint32_t b = 0; b < n; ++b)
+          if (first_use[b] < (1 << 30)) soon |= 1ull << b;
+        int best = next_absorbs(nlogi);
+        const std::pair<uint64_t, bool> alts[] = {{soon, false}, {soon, true}, {Bn, false}};
+        for (const auto& a : alts) {
+          std::vector<int> p2, l2;
+          candidate(a.first, a.second, p2, l2);
+          const int v = next_absorbs(l2);
+          if (v > best) {
+            best = v;
+            nphys = p2;
+            nlogi = l2;
+          }
+        }
+      }
+    }
+    if (dyn) {  // the stored layout is a permutation of the tile's physical bits
+      std::vector<uint8_t> seen(n, 0);
+      for (uint32_t b = 0; b < n; ++b) {
+        const int q = nphys[b];
+        if (q < 0 || q >= (int)n || seen[q] || nlogi[q] != (int)b || ((B >> b) & 1) != ((B >> logi[q]) & 1))
+          throw Error{QR_ERR_UNSUPPORTED, "internal planner error: invalid stored layout"};
+        seen[q] = 1;
+      }
+    }
+    uint64_t nlow_mask = 0;
+    for (int j = 0; j < c; ++j) nlow_mask |= 1ull << nlogi[j];
 
     Stage s;
     s.tile = true;
     s.k = k;
-    for (uint32_t b = 0; b < n; ++b)
-      if (B & (1ull << b)) s.tbits.push_back((int)b);
+    // tile positions: the logical bits at the physical low bits (in physical order), then the
+    // rest by physical position, so that thread slots in position order keep every segment
+    // contiguous
+    for (int j = 0; j < c; ++j) s.tbits.push_back(logi[j]);
+    {
+      std::vector<int> rest;
+      for (uint32_t b = 0; b < n; ++b)
+        if ((B & (1ull << b)) && !(low_mask & (1ull << b))) rest.push_back((int)b);
+      std::sort(rest.begin(), rest.end(), [&](int a, int b2) { return phys[a] < phys[b2]; });
+      s.tbits.insert(s.tbits.end(), rest.begin(), rest.end());
+    }
     int pos_of_bit[64];
     for (int i = 0; i < 64; ++i) pos_of_bit[i] = -1;
     for (int p = 0; p < k; ++p) pos_of_bit[s.tbits[p]] = p;
+    for (int p = 0; p < k; ++p) {
+      s.pload.push_back(phys[s.tbits[p]]);
+      s.pstore.push_back(nphys[s.tbits[p]]);
+    }
     s.bytes = sweep_bytes;
     s.ngates = (int)taken.size();
 
@@ -399,7 +655,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
 
     std::vector<int> rem = taken;
     // first register set: from the non-low tile bits so the load stays coalesced
-    uint64_t R = lookahead_phase_bits(r, nonlow_bits, tile_bits_vec, rem, mk, eligible, cmask, B, low_mask);
+    uint64_t R = lookahead_phase_bits(r, nonlow_bits, tile_bits_vec, rem, mk, eligible, cmask, B, nlow_mask);
     for (int b : nonlow_bits)
       if (__builtin_popcountll(R) < r) R |= 1ull << b;
     map = make_map(R);
@@ -418,16 +674,33 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         const int p = pos_of_bit[b];
         const int sl = p >= 0 ? slot_of_pos(p) : -1;
         if (sl >= nts) t.ns |= 1u << (sl - nts);
-        else t.lane_cmask |= 1ull << b;
+        else t.lane_cmask |= 1ull << phys[b];  // tested on the physical index
       }
     };
     // A remap moves data through LDS; its swizzle keeps the 16 lanes of a quarter-wave on
     // distinct 16-byte LDS slots for both the write (old map) and the read (new map).
-    auto emit_remap = [&](const std::vector<int>& nm) {
+    auto emit_remap = [&](const std::vector<int>& nm_in) {
       TileOpHost t{};
       t.code = TC_REMAP;
       t.swz = (uint32_t)(s.swz.size() / (2 * k));
-      std::vector<uint16_t> table = choose_swizzle(k, nts, map, nm);
+      std::vector<int> nm = nm_in;
+      const int h = split ? choose_split_pos(k, nts, map, nm) : -1;
+      if (h >= 0) {  // keep a register-held split position in the same register slot (free relabel)
+        int os = -1, ns = -1;
+        for (int sl = 0; sl < k; ++sl) {
+          if (map[sl] == h) os = sl;
+          if (nm[sl] == h) ns = sl;
+        }
+        if (os >= nts && ns >= nts && os != ns) std::swap(nm[os], nm[ns]);
+      }
+      std::vector<uint16_t> table;
+      if (h >= 0) {
+        table = choose_swizzle_split(k, nts, map, nm, h);
+        t.flags = 1;
+        t.b = (uint32_t)h;
+      } else {
+        table = choose_swizzle(k, nts, map, nm);
+      }
       // per-slot LDS patterns: old map (write) then new map (read)
       for (int sl = 0; sl < k; ++sl) s.swz.push_back(table[map[sl]]);
       for (int sl = 0; sl < k; ++sl) s.swz.push_back(table[nm[sl]]);
@@ -504,7 +777,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           s.ops.push_back(t);
         } else if (op.kind == FK_DIAG1) {
           t.code = TC_D1;
-          t.a = (uint32_t)op.bits[0];
+          t.a = (uint32_t)phys[op.bits[0]];
           const int p = pos_of_bit[op.bits[0]];
           const int sl = p >= 0 ? slot_of_pos(p) : -1;
           t.b = (sl >= nts) ? (uint32_t)(sl - nts) : 0xFFu;
@@ -528,7 +801,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           if (op.gen == 1 || op.gen == 2) {
             t.a = (uint32_t)(slot_of_pos(pos_of_bit[op.bits[0]]) - nts);
           } else {
-            t.a = (uint32_t)op.bits[0];
+            t.a = (uint32_t)phys[op.bits[0]];
             const int p = op.gen == 3 ? pos_of_bit[op.bits[0]] : -1;
             const int sl = p >= 0 ? slot_of_pos(p) : -1;
             t.b = (sl >= nts) ? (uint32_t)(sl - nts) : 0xFFu;
@@ -546,7 +819,7 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       rem = later;
       if (rem.empty() || hit_swap) continue;
       // next register set: any tile bits
-      uint64_t R2 = lookahead_phase_bits(r, tile_bits_vec, tile_bits_vec, rem, mk, eligible, cmask, B, low_mask);
+      uint64_t R2 = lookahead_phase_bits(r, tile_bits_vec, tile_bits_vec, rem, mk, eligible, cmask, B, nlow_mask);
       for (int b : tile_bits_vec)
         if (__builtin_popcountll(R2) < r) R2 |= 1ull << b;
       if (now.empty() && count_absorb(rem, mk, eligible, R2) == 0)
@@ -572,15 +845,41 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         s.ops.push_back(t);
       }
     }
-    // ---- final layout: low tile positions on the low thread slots for a coalesced store --
+    // ---- final layout: the positions stored at the physical low bits on thread slots 0..c-1
+    // (in physical order) for a coalesced store --------------------------------------------
     bool canon = true;
     for (int sl = 0; sl < c; ++sl)
-      if (map[sl] != sl) canon = false;
+      if (map[sl] != pos_of_bit[nlogi[sl]]) canon = false;
+    bool remapped_last = false;
     if (!canon) {
       uint64_t Rf = 0;
-      for (int b : nonlow_bits)
-        if (__builtin_popcountll(Rf) < r) Rf |= 1ull << b;
-      emit_remap(make_map(Rf));
+      for (int b : s.tbits)
+        if (!(nlow_mask & (1ull << b)) && __builtin_popcountll(Rf) < r) Rf |= 1ull << b;
+      std::vector<int> m;
+      for (int sl = 0; sl < c; ++sl) m.push_back(pos_of_bit[nlogi[sl]]);
+      for (int p = 0; p < k; ++p)
+        if (!(Rf & (1ull << s.tbits[p])) && !(nlow_mask & (1ull << s.tbits[p]))) m.push_back(p);
+      for (int p = 0; p < k; ++p)
+        if (Rf & (1ull << s.tbits[p])) m.push_back(p);
+      emit_remap(m);
+      remapped_last = true;
+    }
+    if (s.pstore != s.pload) {
+      // The last map addresses the store layout; ops before it use the load layout. A thread
+      // then stores where other threads of the tile loaded, so every load must be done first:
+      // a remap's barrier orders them (loads all precede a stage's first remap). A stage with
+      // no remap gets one that keeps the map (a round trip through LDS); otherwise a relabel.
+      bool any_remap = false;
+      for (const auto& t : s.ops) any_remap = any_remap || t.code == TC_REMAP;
+      if (!any_remap) {
+        emit_remap(map);
+      } else if (!remapped_last) {
+        TileOpHost t{};
+        t.code = TC_SWAPSLOTS;
+        t.a = (uint32_t)push_map(map);
+        s.ops.push_back(t);
+      }
+      s.store_map = (int)(s.maps.size() / k) - 1;
     }
     {  // plan-time values of the coefficients (the CPU model reads them)
       std::vector<cd> c;
@@ -590,6 +889,14 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
     }
     stages.push_back(std::move(s));
     remaining = left;
+    if (restore_only && nphys != phys) {
+      for (uint32_t b = 0; b < n; ++b)
+        if (nphys[b] != (int)b) throw Error{QR_ERR_UNSUPPORTED, "internal planner error: the layout was not restored"};
+    } else if (restore_only) {
+      throw Error{QR_ERR_UNSUPPORTED, "internal planner error: a restoring sweep changed nothing"};
+    }
+    phys = nphys;
+    logi = nlogi;
   }
   return stages;
 }

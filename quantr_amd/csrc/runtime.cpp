@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -43,6 +44,18 @@ PlanParams plan_params(const qr_state* st) {
   if ((st->opt.reg_bits == 5 || st->opt.reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = (int)st->opt.reg_bits;
   pp.low_bits = st->opt.low_bits ? (int)st->opt.low_bits : (st->prec == 0 ? 3 : 4);
   pp.max_ops = stage_op_limit(st, pp.tile_bits);
+  // tiles of 64 KB or more go through LDS in two halves: three complex128 workgroups of
+  // 2^12 amplitudes then share a CU instead of two (measured 7.5 against 7.8 ms per 30-qubit
+  // sweep); QR_LDS_SPLIT=0 or 1 overrides
+  // (complex64 tiles of 2^13 run as fast without: 102.0 against 104.3 ms per 30-qubit step)
+  pp.lds_split = (st->prec == 0 && (1ull << pp.tile_bits) * st->bytes_per_amp() >= 65536) ? 1 : 0;
+  if (const char* e = std::getenv("QR_LDS_SPLIT")) pp.lds_split = e[0] == '1' ? 1 : 0;
+  // moving layout: each stage stores at the physical low bits (the contiguous segments of
+  // every tile) bits the next stage uses, instead of always the same wires; QR_DYN_LOW=0 keeps
+  // the identity layout
+  pp.dyn_low = 1;
+  if (const char* e = std::getenv("QR_DYN_LOW"))
+    if (e[0] == '0') pp.dyn_low = 0;
   return pp;
 }
 
@@ -87,8 +100,8 @@ std::vector<uint8_t> pack_blob(const std::vector<FastOp>& ops, const std::vector
       loc[si].coefs = append(c.data(), c.size() * sizeof(double2));
       align(4);
       std::vector<uint32_t> m(s.maps.size());
-      for (size_t i = 0; i < m.size(); ++i)
-        m[i] = (uint32_t)s.tbits[s.maps[i]] | ((uint32_t)s.maps[i] << 8);
+      for (size_t i = 0; i < m.size(); ++i)  // physical bit | tile position << 8
+        m[i] = (uint32_t)s.phys_bit((int)(i / s.k), s.maps[i]) | ((uint32_t)s.maps[i] << 8);
       loc[si].maps = append(m.data(), m.size() * sizeof(uint32_t));
       std::vector<uint32_t> z(s.swz.begin(), s.swz.end());
       if (z.empty()) z.push_back(0);
@@ -127,7 +140,7 @@ TileArgs tile_args_at(uint32_t n, const Stage& s, const uint8_t* dblob, const Lo
   a.nops = (uint32_t)s.ops.size();
   a.ntiles = 1ull << (n - s.k);
   uint64_t tm = 0;
-  for (int b : s.tbits) tm |= 1ull << b;
+  for (int b : s.pload) tm |= 1ull << b;  // physical bits of the tile (the same at the store)
   a.nontile_mask = (n >= 64 ? ~0ull : ((1ull << n) - 1)) & ~tm;
   return a;
 }
@@ -205,7 +218,7 @@ std::shared_ptr<CachedPlan> get_plan_for(uint32_t n, int prec, const std::vector
                                          bool fused, bool dual, bool persistent) {
   std::vector<uint64_t> key{n, (uint64_t)prec, (uint64_t)pp.tile_bits, (uint64_t)pp.reg_bits,
                             (uint64_t)pp.low_bits, (uint64_t)pp.max_ops, (uint64_t)fused, (uint64_t)dual,
-                            (uint64_t)persistent};
+                            (uint64_t)persistent, (uint64_t)pp.lds_split, (uint64_t)pp.dyn_low};
   key.reserve(8 + 3 * ops.size());
   for (const FastOp& op : ops) plan_key_append(op, key);
   {

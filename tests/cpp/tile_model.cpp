@@ -63,14 +63,21 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
   const int K = s.k, NT = K - RB;
   const uint64_t ntiles = 1ull << (n - K);
   uint64_t tm = 0;
-  for (int b : s.tbits) tm |= 1ull << b;
+  for (int b : s.pload) tm |= 1ull << b;
   std::vector<int> nontile;
   for (uint32_t b = 0; b < n; ++b)
     if (!(tm & (1ull << b))) nontile.push_back((int)b);
   const int T = 1 << NT, R = 1 << RB;
+  // a stage that stores in another layout than it loads needs a barrier (a remap) between
+  // every load and the stores, since threads then store where other threads loaded
+  if (s.pstore != s.pload) {
+    bool any_remap = false;
+    for (const TileOpHost& op : s.ops) any_remap = any_remap || op.code == TC_REMAP;
+    if (!any_remap) throw Error{QR_ERR_HIP, "model: layout moves in a stage without a barrier"};
+  }
   std::vector<cdv> lds(1u << K);
   std::vector<uint8_t> lds_written(1u << K);
-  auto gbit = [&](int mapi, int slot) { return s.tbits[s.maps[(size_t)mapi * K + slot]]; };
+  auto gbit = [&](int mapi, int slot) { return s.phys_bit(mapi, s.maps[(size_t)mapi * K + slot]); };
   for (uint64_t blk = 0; blk < ntiles; ++blk) {
     uint64_t base = 0;
     for (size_t i = 0; i < nontile.size(); ++i) base |= ((blk >> i) & 1ull) << nontile[i];
@@ -163,6 +170,53 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
             if ((m >> j) & 1) x ^= Q[NT + j];
           return x;
         };
+        if (op.flags & 1) {
+          // split remap: two rounds through half the LDS (the compiled kernels' exchange)
+          const int h = (int)op.b;
+          int osl = -1, nsl = -1;
+          for (int sl = 0; sl < K; ++sl) {
+            if (s.maps[(size_t)cur_map * K + sl] == h) osl = sl;
+            if (s.maps[(size_t)op.a * K + sl] == h) nsl = sl;
+          }
+          if (osl < 0 || nsl < 0) throw Error{QR_ERR_HIP, "model: split position not in the maps"};
+          const uint32_t hm = (1u << (K - 1)) - 1;
+          auto side = [&](int slot, int t, int m) {  // which round amplitude (t, m) belongs to
+            return slot < NT ? (int)((t >> slot) & 1) : (int)((m >> (slot - NT)) & 1);
+          };
+          for (auto* rs : sets) {
+            std::vector<std::vector<cdv>>& reg = *rs;
+            std::vector<std::vector<cdv>> tmpv(T, std::vector<cdv>(R));
+            std::vector<std::vector<uint8_t>> used(T, std::vector<uint8_t>(R, 0));
+            for (int rho = 0; rho < 2; ++rho) {
+              std::fill(lds_written.begin(), lds_written.end(), 0);
+              for (int t = 0; t < T; ++t)
+                for (int m = 0; m < R; ++m) {
+                  if (side(osl, t, m) != rho) continue;
+                  const uint32_t at = lofs(0, t, m) & hm;
+                  if (lds_written[at]) throw Error{QR_ERR_HIP, "model: LDS write collision (split remap)"};
+                  lds_written[at] = 1;
+                  lds[at] = reg[t][m];
+                }
+              for (int t = 0; t < T; ++t)
+                for (int m = 0; m < R; ++m) {
+                  if (side(nsl, t, m) != rho) continue;
+                  const uint32_t at = lofs(1, t, m) & hm;
+                  if (!lds_written[at]) throw Error{QR_ERR_HIP, "model: read of an unwritten LDS slot (split remap)"};
+                  if (rho == 0 && split_read_needs_tmp(osl, nsl, NT, m)) {
+                    tmpv[t][m] = lds[at];
+                    used[t][m] = 1;
+                  } else {
+                    reg[t][m] = lds[at];
+                  }
+                }
+            }
+            for (int t = 0; t < T; ++t)
+              for (int m = 0; m < R; ++m)
+                if (used[t][m]) reg[t][m] = tmpv[t][m];
+          }
+          cur_map = (int)op.a;
+          continue;
+        }
         for (auto* rs : sets) {
           std::vector<std::vector<cdv>>& reg = *rs;
           std::fill(lds_written.begin(), lds_written.end(), 0);
@@ -204,6 +258,35 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
     if ((reg_bits == 5 || reg_bits == 3) && pp.tile_bits >= 10) pp.reg_bits = reg_bits;
     pp.low_bits = low_bits;
     if (pp.tile_bits == (int)n) pp.max_ops = kWholeStateStageOps;
+    auto stages = plan_fused(n, ops, pp, 16, false);
+    std::vector<cdv> psi(1ull << n);
+    std::memcpy(psi.data(), state, psi.size() * 16);
+    for (const Stage& s : stages) {
+      if (s.tile) run_tile(n, psi, s, pp.reg_bits);
+      else apply_standalone(n, psi, ops[s.standalone]);
+    }
+    std::memcpy(state, psi.data(), psi.size() * 16);
+    if (sweeps_out) *sweeps_out = stages.size();
+    return 0;
+  } catch (const Error& e) {
+    set_error(e.msg);
+    return 1;
+  }
+}
+
+// model_run with every plan option: remaps split into two rounds through half the LDS
+// (PlanParams::lds_split) and the moving layout (PlanParams::dyn_low).
+int model_run_opts(uint32_t n, double* state, const qr_program* prog, int tile_bits, int low_bits, int reg_bits,
+                   int max_ops, int lds_split, int dyn_low, uint64_t* sweeps_out) {
+  try {
+    auto ops = lower_fast(n, lower_exact(n, prog), false);
+    PlanParams pp;
+    pp.tile_bits = std::min<int>(tile_bits, (int)n);
+    pp.reg_bits = std::min(reg_bits, pp.tile_bits);
+    pp.low_bits = low_bits;
+    pp.max_ops = max_ops;
+    pp.lds_split = lds_split;
+    pp.dyn_low = dyn_low;
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);

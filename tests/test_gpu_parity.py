@@ -100,6 +100,67 @@ def test_tile_widths(tile_bits):
     assert np.max(np.abs(got - want)) < TOL_FUSED
 
 
+@pytest.mark.parametrize("low_bits", [3, 4])
+def test_split_lds_tiles(low_bits):
+    """Tiles of 2^13 complex128 amplitudes (128 KB) exchange through half the LDS in two rounds
+    in compiled stages; the interpreter runs the same plans through the whole tile in LDS."""
+    rng = np.random.default_rng(1300 + low_bits)
+    for trial in range(6):
+        n = int(rng.integers(13, 18))
+        gates, customs = H.random_gates(n, 150, rng, with_custom=(trial % 3 == 0))
+        p = H.to_program(gates, customs)
+        want = H.oracle_simulate(n, p)
+        for jit in (1, 0):
+            say("split tiles trial", trial, "n", n, "low bits", low_bits, "jit", jit)
+            st = _capi.State(n)
+            st.set_options(fused=1, tile_bits=13, low_bits=low_bits, jit=jit)
+            st.apply(p)
+            got = st.amplitudes()
+            st.close()
+            assert np.max(np.abs(got - want)) < TOL_FUSED, (trial, jit)
+    n = 18
+    p = W.to_program(W.brickwork(n, depth=12, seed=18))
+    want = H.oracle_simulate(n, p, threads=8)
+    st = _capi.State(n)
+    st.set_options(fused=1, tile_bits=13, low_bits=low_bits, jit=1)
+    st.apply(p)
+    assert np.max(np.abs(st.amplitudes() - want)) < TOL_FUSED
+    st.close()
+
+
+@pytest.mark.parametrize("tile_bits,low_bits", [(12, 4), (13, 4), (12, 3)])
+def test_moving_layout(tile_bits, low_bits, monkeypatch):
+    """Plans whose stages move bits into the physical low bits (restored to the identity before
+    standalone ops and at the end), compiled and interpreted, against the oracle."""
+    monkeypatch.setenv("QR_DYN_LOW", "1")  # the default; set explicitly for clarity
+    rng = np.random.default_rng(1500 + tile_bits * 10 + low_bits)
+    for trial in range(6):
+        n = int(rng.integers(tile_bits + 1, tile_bits + 5))
+        gates, customs = H.random_gates(n, 150, rng, with_custom=(trial % 2 == 0))
+        p = H.to_program(gates, customs)
+        want = H.oracle_simulate(n, p)
+        for prec, tol in ((_capi.QR_C128, TOL_FUSED), (_capi.QR_C64, TOL_C64)):
+            for jit in (1, 0):
+                say("moving layout trial", trial, "n", n, "prec", prec, "jit", jit)
+                st = _capi.State(n, prec)
+                st.set_options(fused=1, tile_bits=tile_bits, low_bits=low_bits, jit=jit)
+                st.apply(p)
+                got = st.amplitudes()
+                st.close()
+                assert np.max(np.abs(got - want)) < tol, (trial, prec, jit)
+
+
+def test_identity_layout_knob(monkeypatch):
+    """QR_DYN_LOW=0 plans with the identity layout; same results."""
+    monkeypatch.setenv("QR_DYN_LOW", "0")
+    rng = np.random.default_rng(1600)
+    n = 15
+    gates, customs = H.random_gates(n, 120, rng, with_custom=True)
+    p = H.to_program(gates, customs)
+    got = run_gpu(n, p, fused=1, jit=1)
+    assert np.max(np.abs(got - H.oracle_simulate(n, p))) < TOL_FUSED
+
+
 def test_complex64_random():
     rng = np.random.default_rng(404)
     for trial in range(10):

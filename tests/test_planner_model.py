@@ -268,3 +268,60 @@ def test_model_custom_diagonal_zero_entries(diag, controls):
         got, _ = run_model(n, p, tile_bits)
         assert np.all(np.isfinite(got)), (diag, controls, tile_bits)
         assert np.max(np.abs(got - want)) < 1e-12, (diag, controls, tile_bits)
+
+
+def run_model_opts(n, program, tile_bits, low_bits, reg_bits=4, max_ops=0, lds_split=0, dyn_low=0):
+    lib = model()
+    lib.model_run_opts.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                   C.c_int, C.POINTER(C.c_uint64)]
+    psi = np.zeros(1 << n, dtype=np.complex128)
+    psi[0] = 1.0
+    p = program.c_struct()
+    sw = C.c_uint64()
+    rc = lib.model_run_opts(n, psi.ctypes.data, C.byref(p), tile_bits, low_bits, reg_bits, max_ops, lds_split,
+                            dyn_low, C.byref(sw))
+    assert rc == 0, lib.model_error()
+    return psi, sw.value
+
+
+def run_model_split(n, program, tile_bits, low_bits, reg_bits=4, max_ops=0):
+    return run_model_opts(n, program, tile_bits, low_bits, reg_bits, max_ops, lds_split=1)
+
+
+@pytest.mark.parametrize("tile_bits,low_bits,reg_bits", [(13, 4, 4), (13, 3, 4), (12, 3, 4), (10, 3, 3), (11, 4, 5)])
+def test_model_split_remaps(tile_bits, low_bits, reg_bits):
+    """Remaps through half the LDS in two rounds (the split position's bit is the top LDS
+    address bit; first-round reads still to be overwritten go through temporaries): every
+    round writes and reads each LDS slot once, and the states match the oracle."""
+    rng = np.random.default_rng(3300 + tile_bits * 10 + low_bits + reg_bits)
+    for trial in range(6):
+        n = int(rng.integers(tile_bits, tile_bits + 3))
+        gates, customs = H.random_gates(n, 90, rng, with_custom=(trial % 3 == 0))
+        p = H.to_program(gates, customs)
+        got, _ = run_model_split(n, p, tile_bits, low_bits, reg_bits)
+        want = H.oracle_simulate(n, p)
+        assert np.max(np.abs(got - want)) < 1e-12, (trial, n)
+    n = tile_bits + 2
+    p = W.to_program(W.brickwork(n, depth=8, seed=5))
+    got, _ = run_model_split(n, p, tile_bits, low_bits, reg_bits)
+    assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12
+
+
+@pytest.mark.parametrize("tile_bits,low_bits,lds_split", [(12, 4, 1), (12, 3, 0), (10, 4, 0), (6, 2, 0), (13, 4, 1)])
+def test_model_moving_layout(tile_bits, low_bits, lds_split):
+    """The moving layout: stages store bits the next stage uses at the physical low bits, the
+    layout is restored before standalone ops and at the end; states match the oracle."""
+    rng = np.random.default_rng(4400 + tile_bits * 10 + low_bits)
+    for trial in range(8):
+        n = int(rng.integers(tile_bits + 1, tile_bits + 5))
+        gates, customs = H.random_gates(n, 120, rng, with_custom=(trial % 2 == 0))
+        p = H.to_program(gates, customs)
+        got, _ = run_model_opts(n, p, tile_bits, low_bits, lds_split=lds_split, dyn_low=1)
+        want = H.oracle_simulate(n, p)
+        assert np.max(np.abs(got - want)) < 1e-12, (trial, n)
+    n = tile_bits + 4
+    p = W.to_program(W.brickwork(n, depth=10, seed=9))
+    got, sweeps = run_model_opts(n, p, tile_bits, low_bits, lds_split=lds_split, dyn_low=1)
+    assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12
+    _, fixed = run_model_opts(n, p, tile_bits, low_bits, lds_split=lds_split, dyn_low=0)
+    assert sweeps <= fixed + 1, (sweeps, fixed)
