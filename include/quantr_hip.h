@@ -27,8 +27,9 @@
 extern "C" {
 #endif
 
-/* 2: qr_options gained stage_ops (a struct of 8 words; version 1 had 7). */
-#define QR_ABI_VERSION 2u
+/* 2: qr_options gained stage_ops (a struct of 8 words; version 1 had 7).
+ * 3: qr_plan_stats gained max_stage_gates and restore_sweeps. */
+#define QR_ABI_VERSION 3u
 
 typedef enum qr_status {
   QR_OK = 0,
@@ -301,10 +302,13 @@ qr_status qr_profile_read(qr_state* state, qr_profile_entry* entries, uint64_t c
                           uint64_t* count_out);
 /* Statistics of the last qr_apply_program: sweeps launched and algorithmic bytes. */
 typedef struct qr_plan_stats {
-  uint64_t gates;        /* non-identity gates in the program          */
-  uint64_t sweeps;       /* kernel launches that touch the state        */
-  uint64_t tile_sweeps;  /* of which fused tile sweeps                  */
-  double bytes;          /* sum over sweeps of 2 * 2^n * B * f_touch    */
+  uint64_t gates;           /* non-identity gates in the program          */
+  uint64_t sweeps;          /* kernel launches that touch the state        */
+  uint64_t tile_sweeps;     /* of which fused tile sweeps                  */
+  double bytes;             /* sum over sweeps of 2 * 2^n * B * f_touch    */
+  uint64_t max_stage_gates; /* most gates fused into one tile sweep        */
+  uint64_t restore_sweeps;  /* gate-free tile sweeps that restore the
+                               state's bit order (moving layout)           */
 } qr_plan_stats;
 qr_status qr_state_last_plan(const qr_state* state, qr_plan_stats* out);
 

@@ -54,7 +54,7 @@ class QrBatchOptions(C.Structure):
 
 class QrPlanStats(C.Structure):
     _fields_ = [("gates", C.c_uint64), ("sweeps", C.c_uint64), ("tile_sweeps", C.c_uint64),
-                ("bytes", C.c_double)]
+                ("bytes", C.c_double), ("max_stage_gates", C.c_uint64), ("restore_sweeps", C.c_uint64)]
 
 
 class QrProfileEntry(C.Structure):
@@ -107,7 +107,7 @@ _SIGNATURES = [
 EXPORTED = [s[0] for s in _SIGNATURES]
 
 _lib = None
-ABI_VERSION = 2  # QR_ABI_VERSION of include/quantr_hip.h; the structures above follow it
+ABI_VERSION = 3  # QR_ABI_VERSION of include/quantr_hip.h; the structures above follow it
 
 
 class QrError(RuntimeError):
@@ -308,7 +308,8 @@ class State:
     def last_plan(self):
         s = QrPlanStats()
         check(lib().qr_state_last_plan(self._h, C.byref(s)))
-        return {"gates": s.gates, "sweeps": s.sweeps, "tile_sweeps": s.tile_sweeps, "bytes": s.bytes}
+        return {"gates": s.gates, "sweeps": s.sweeps, "tile_sweeps": s.tile_sweeps, "bytes": s.bytes,
+                "max_stage_gates": s.max_stage_gates, "restore_sweeps": s.restore_sweeps}
 
     def profile_enable(self, on=True):
         check(lib().qr_profile_enable(self._h, 1 if on else 0))

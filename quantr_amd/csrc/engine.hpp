@@ -95,6 +95,21 @@ struct FastOp {
   uint64_t read_mask() const;  // controls and diagonal bits
 };
 
+// One Pauli-sum term as bit masks: coeff * i^ny * X^xm Z^zm (index bits, wire 0 = MSB).
+struct PauliMasks {
+  double coeff;
+  uint64_t xm, zm;
+  int ny;
+};
+// Checked observable terms (wires in range, no repeated wire, X/Y/Z only). Throws Error.
+std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o);
+PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t);
+// A term as the device's Z-mask term: the coefficient times i^ny.
+PauliZTerm z_term(const PauliMasks& tm);
+// The observable as device tables: terms grouped by X mask (i^ny folded into coefficients).
+void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
+                  std::vector<PauliZTerm>& flat);
+
 // Validates and lowers a program. Throws Error.
 void validate_program(uint32_t n, const qr_program* p);
 std::vector<ExactGate> lower_exact(uint32_t n, const qr_program* p);

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <sstream>
 
 #include "engine.hpp"
@@ -180,8 +181,12 @@ void validate_program(uint32_t n, const qr_program* p) {
     if (expect >= 0 && (int)op.ncontrols != expect)
       throw arg_error("gate " + std::string(kind_name(op.kind)) + " needs " + std::to_string(expect) +
                       " control wire(s), got " + std::to_string(op.ncontrols));
+    if (op.kind == QR_GATE_CUSTOM && op.ncontrols + 1ull > 16)
+      throw Error{QR_ERR_UNSUPPORTED, "custom gates act on at most 16 wires"};
+    // ranges compared without overflow: offset <= size and count <= size - offset
+    auto in_range = [](uint64_t off, uint64_t count, uint64_t size) { return off <= size && count <= size - off; };
     if (op.ncontrols) {
-      if (!p->controls || op.controls_offset + op.ncontrols > p->ncontrols)
+      if (!p->controls || !in_range(op.controls_offset, op.ncontrols, p->ncontrols))
         throw arg_error("control list of gate " + std::to_string(g) + " is out of range");
     }
     const uint32_t* c = op.ncontrols ? p->controls + op.controls_offset : nullptr;
@@ -207,11 +212,10 @@ void validate_program(uint32_t n, const qr_program* p) {
       }
     if (op.kind == QR_GATE_CUSTOM) {
       const uint32_t k = op.ncontrols + 1;
-      if (k > 16) throw Error{QR_ERR_UNSUPPORTED, "custom gates act on at most 16 wires"};
       const uint64_t dim = 1ull << k;
-      if (!p->some || op.some_offset + dim > p->nsome)
+      if (!p->some || !in_range(op.some_offset, dim, p->nsome))
         throw arg_error("custom gate " + std::to_string(g) + ": Some/None table out of range");
-      if (!p->images || op.image_offset + dim * dim * 2 > p->nimages)
+      if (!p->images || !in_range(op.image_offset, dim * dim * 2, p->nimages))
         throw arg_error("custom gate " + std::to_string(g) + ": image table out of range (needs " +
                         std::to_string(dim) + " images of " + std::to_string(dim) + " amplitudes)");
     }
@@ -426,6 +430,62 @@ FastOp adjoint(const FastOp& op) {
   }
   classify(a);
   return a;
+}
+
+// ---- observables (Pauli sums) ---------------------------------------------------------
+
+PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t) {
+  const qr_pauli_term& pt = o->terms[t];
+  if (pt.nfactors && (!o->wires || !o->paulis || pt.offset > o->nfactors || pt.nfactors > o->nfactors - pt.offset))
+    throw Error{QR_ERR_ARG, "observable term " + std::to_string(t) + " is out of range"};
+  PauliMasks tm{pt.coeff, 0, 0, 0};
+  uint64_t seen = 0;
+  for (uint32_t f = 0; f < pt.nfactors; ++f) {
+    const uint32_t w = o->wires[pt.offset + f];
+    if (w >= n) throw Error{QR_ERR_ARG, "observable wire " + std::to_string(w) + " is out of range"};
+    const uint64_t b = 1ull << (n - 1 - w);
+    if (seen & b) throw Error{QR_ERR_ARG, "observable term repeats wire " + std::to_string(w)};
+    seen |= b;
+    switch (o->paulis[pt.offset + f]) {
+      case 'X': case 1: tm.xm |= b; break;
+      case 'Y': case 2: tm.xm |= b; tm.zm |= b; tm.ny++; break;
+      case 'Z': case 3: tm.zm |= b; break;
+      case 'I': case 0: break;
+      default: throw Error{QR_ERR_ARG, "observable pauli must be X, Y or Z"};
+    }
+  }
+  return tm;
+}
+
+std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o) {
+  if (!o) throw Error{QR_ERR_ARG, "observable is NULL"};
+  if (o->nterms && !o->terms) throw Error{QR_ERR_ARG, "observable terms are NULL"};
+  std::vector<PauliMasks> out;
+  for (uint64_t t = 0; t < o->nterms; ++t) out.push_back(parse_term(n, o, t));
+  return out;
+}
+
+PauliZTerm z_term(const PauliMasks& tm) {
+  double cr = tm.coeff, ci = 0.0;
+  for (int q = 0; q < (tm.ny & 3); ++q) {  // i^ny folded into the coefficient
+    const double t = cr;
+    cr = -ci;
+    ci = t;
+  }
+  return PauliZTerm{tm.zm, cr, ci};
+}
+
+void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
+                  std::vector<PauliZTerm>& flat) {
+  // terms grouped by X mask; Z masks and i^ny folded into per-term complex coefficients
+  std::map<uint64_t, std::vector<PauliZTerm>> by_x;
+  for (const auto& tm : terms) by_x[tm.xm].push_back(z_term(tm));
+  groups.clear();
+  flat.clear();
+  for (const auto& g : by_x) {
+    groups.push_back(PauliGroup{g.first, (uint32_t)flat.size(), (uint32_t)g.second.size()});
+    flat.insert(flat.end(), g.second.begin(), g.second.end());
+  }
 }
 
 }  // namespace qr

@@ -255,9 +255,11 @@ std::shared_ptr<CachedPlan> get_plan_for(uint32_t n, int prec, const std::vector
   return cp;
 }
 
-// Compiles (once) the tile stages of a cached plan and resolves their kernels.
+// Compiles (once) the tile stages of a cached plan and resolves their kernels. Only this
+// plan's lock is held while compiling, so other threads keep finding and using other plans;
+// threads that use the plan take the same lock before reading its kernels.
 void prepare_compiled(CachedPlan& cp, int prec, int RB, bool persistent, bool dual) {
-  std::lock_guard<std::mutex> lk(g_plan_mu);
+  std::lock_guard<std::mutex> lk(cp.compile_mu);
   if (!cp.fns.empty() || cp.stages.empty()) return;
   cp.src = jit_stage_sources(cp.stages, prec, RB, persistent, dual);
   jit_prepare(cp.src);
@@ -306,7 +308,11 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
     launch_standalone(st, st->amp, ops[s.standalone], P, si);
   }
   ps.gates = 0;
-  for (const Stage& s : stages) ps.gates += s.ngates;
+  for (const Stage& s : stages) {
+    ps.gates += s.ngates;
+    if (s.tile) ps.max_stage_gates = std::max<uint64_t>(ps.max_stage_gates, (uint64_t)s.ngates);
+    if (s.tile && s.ngates == 0) ps.restore_sweeps++;
+  }
   if (stats) *stats = ps;
 }
 
@@ -456,47 +462,6 @@ double norm_sqr(qr_state* st) {
   return sum_partials_host(h);
 }
 
-PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t) {
-  const qr_pauli_term& pt = o->terms[t];
-  if (pt.nfactors && (!o->wires || !o->paulis || pt.offset + pt.nfactors > o->nfactors))
-    throw Error{QR_ERR_ARG, "observable term " + std::to_string(t) + " is out of range"};
-  PauliMasks tm{pt.coeff, 0, 0, 0};
-  uint64_t seen = 0;
-  for (uint32_t f = 0; f < pt.nfactors; ++f) {
-    const uint32_t w = o->wires[pt.offset + f];
-    if (w >= n) throw Error{QR_ERR_ARG, "observable wire " + std::to_string(w) + " is out of range"};
-    const uint64_t b = 1ull << (n - 1 - w);
-    if (seen & b) throw Error{QR_ERR_ARG, "observable term repeats wire " + std::to_string(w)};
-    seen |= b;
-    switch (o->paulis[pt.offset + f]) {
-      case 'X': case 1: tm.xm |= b; break;
-      case 'Y': case 2: tm.xm |= b; tm.zm |= b; tm.ny++; break;
-      case 'Z': case 3: tm.zm |= b; break;
-      case 'I': case 0: break;
-      default: throw Error{QR_ERR_ARG, "observable pauli must be X, Y or Z"};
-    }
-  }
-  return tm;
-}
-
-std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o) {
-  if (!o) throw Error{QR_ERR_ARG, "observable is NULL"};
-  if (o->nterms && !o->terms) throw Error{QR_ERR_ARG, "observable terms are NULL"};
-  std::vector<PauliMasks> out;
-  for (uint64_t t = 0; t < o->nterms; ++t) out.push_back(parse_term(n, o, t));
-  return out;
-}
-
-PauliZTerm z_term(const PauliMasks& tm) {
-  double cr = tm.coeff, ci = 0.0;
-  for (int q = 0; q < (tm.ny & 3); ++q) {  // i^ny folded into the coefficient
-    const double t = cr;
-    cr = -ci;
-    ci = t;
-  }
-  return PauliZTerm{tm.zm, cr, ci};
-}
-
 // <psi|O|psi> in one pass over the state (terms grouped by X mask), partials summed on the host.
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms) {
   if (terms.empty()) return 0.0;
@@ -519,19 +484,6 @@ double expectation(qr_state* st, const std::vector<PauliMasks>& terms) {
   return sum_partials_host(h);
 }
 
-
-void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
-                  std::vector<PauliZTerm>& flat) {
-  // terms grouped by X mask; Z masks and i^ny folded into per-term complex coefficients
-  std::map<uint64_t, std::vector<PauliZTerm>> by_x;
-  for (const auto& tm : terms) by_x[tm.xm].push_back(z_term(tm));
-  groups.clear();
-  flat.clear();
-  for (const auto& g : by_x) {
-    groups.push_back(PauliGroup{g.first, (uint32_t)flat.size(), (uint32_t)g.second.size()});
-    flat.insert(flat.end(), g.second.begin(), g.second.end());
-  }
-}
 
 void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const std::vector<PauliMasks>& terms,
                       DevBuf& scratch, hipStream_t s) {

@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -46,12 +47,6 @@ struct ProfRec {
   double bytes;
 };
 
-// One Pauli-sum term as bit masks: coeff * i^ny * X^xm Z^zm (index bits, wire 0 = MSB).
-struct PauliMasks {
-  double coeff;
-  uint64_t xm, zm;
-  int ny;
-};
 
 }  // namespace qr
 
@@ -140,6 +135,7 @@ struct PackedPlan {
 // A plan shared by every program of one structure, with its compiled stages.
 struct CachedPlan {
   std::vector<Stage> stages;
+  std::mutex compile_mu;           // held while this plan's stages compile (not the cache lock)
   std::vector<std::string> src;    // compiled-stage sources (filled on first compiled use)
   std::vector<hipFunction_t> fns;  // their kernels
   std::atomic<uint64_t> runs{0};  // applications (states on several threads may share a plan)
@@ -172,17 +168,9 @@ double sum_partials_host(const double* v, size_t count);
 // whose left-fold cumulative probability exceeds us[s], or UINT64_MAX (None).
 void sample_exact_parallel(qr_state* st, const std::vector<double>& us, std::vector<uint64_t>& res);
 double norm_sqr(qr_state* st);
-std::vector<PauliMasks> parse_observable(uint32_t n, const qr_observable* o);
-// Term t of an observable as masks (checked: wires in range, no repeated wire, X/Y/Z only).
-PauliMasks parse_term(uint32_t n, const qr_observable* o, uint64_t t);
-// A term as the device's Z-mask term: the coefficient times i^ny.
-PauliZTerm z_term(const PauliMasks& tm);
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms);
 // The same value through fixed device and host arrays: no allocation (noalloc.cpp).
 double expectation_noalloc(qr_state* st, const qr_observable* o);
-// The observable as device tables: terms grouped by X mask (i^ny folded into coefficients).
-void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
-                  std::vector<PauliZTerm>& flat);
 // lam = O psi for an observable, on stream s (one pass over the state).
 void apply_observable(int prec, void* lam, const void* psi, uint64_t dim, const std::vector<PauliMasks>& terms,
                       DevBuf& scratch, hipStream_t s);

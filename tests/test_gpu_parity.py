@@ -258,14 +258,22 @@ def test_balanced_plan_25q_vs_exact():
     planned and the lowest estimated duration kept); a caller-set cap is honoured as given.
     Both match the reference-order mode."""
     n = 25
-    gates = W.brickwork(n, depth=24, seed=77)
+    gates = W.brickwork(n, depth=40, seed=77)
     p = W.to_program(gates)
     say("balanced 25q exact")
     want = run_gpu(n, p, fused=0)
     for jit in (0, 1):
         say("balanced 25q fused jit=%d" % jit)
-        got = run_gpu(n, p, fused=1, jit=jit)
+        st = _capi.State(n)
+        st.set_options(fused=1, jit=jit)
+        st.apply(p)
+        got = st.amplitudes()
+        plan = st.last_plan()
+        st.close()
         assert np.max(np.abs(got - want)) < TOL_FUSED, jit
+        # the balancing chose a cap: no stage holds more gates than the largest cap tried (160)
+        say("balanced 25q: largest stage %d gates, %d sweeps" % (plan["max_stage_gates"], plan["sweeps"]))
+        assert 0 < plan["max_stage_gates"] <= 160
     say("balanced 25q stage_ops=7")
     st = _capi.State(n)
     st.set_options(fused=1, jit=0, stage_ops=7)
