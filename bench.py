@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-serving leg")
     ap.add_argument("--batch-instances", type=int, default=512)
     ap.add_argument("--batch-streams", type=int, default=0, help="streams inside the batch graph (0 = 8)")
+    ap.add_argument("--kernels", action="store_true",
+                    help="also time each standalone kernel (reference-order gates, custom tables, sampler, "
+                         "reductions, streaming ceiling)")
     ap.add_argument("--capacity", action="store_true",
                     help="also run the 33-qubit complex128 and 34-qubit complex64 capacity circuits")
     return ap.parse_args()
@@ -122,13 +125,14 @@ def bench_simulate(args):
     log("bench: measure_all 500 shots %.2f ms, 100000 shots %.2f ms (block masses %.0f GB/s, exact sums %.0f GB/s)"
         % (sampling["500"]["ms"], sampling["100000"]["ms"], sampling["500"]["block_mass_gbs"],
            sampling["500"]["block_exact_sums_gbs"]))
+    stream_ms = st.measure_stream(5)
     st.close()
     ms = (t1 - t0) * 1000.0 / args.steps
     init_bytes = float(1 << n) * B
     step_bytes = plan["bytes"] + init_bytes
     log("bench: %.3f ms/step, %d sweeps (%d tile), norm-1 = %.2e" % (ms, plan["sweeps"], plan["tile_sweeps"], norm - 1))
     return {"ms": ms, "plan": plan, "prof": prof, "step_bytes": step_bytes, "gates": len(gates), "n": n,
-            "B": B, "norm_err": abs(norm - 1.0), "sampling": sampling}
+            "B": B, "norm_err": abs(norm - 1.0), "sampling": sampling, "stream_ms": stream_ms}
 
 
 def bench_adjoint(args):
@@ -378,9 +382,106 @@ def cpu_baseline(n, depth):
     t = time.perf_counter() - t0
     del psi
     rate = sample / t
-    return {"value": rate, "unit": "gates/s", "cores": threads, "kind": "port",
-            "sample": "gates 3..%d of the %d-qubit c128 depth-%d brickwork (%.1f GB state), oracle "
-                      "state vector (oracle/quantr_oracle.cpp, OpenMP)" % (2 + sample, n, depth, state_gb)}
+    out = {"value": rate, "unit": "gates/s", "cores": threads, "kind": "port",
+           "sample": "gates 3..%d of the %d-qubit c128 depth-%d brickwork (%.1f GB state), oracle "
+                     "state vector (oracle/quantr_oracle.cpp, OpenMP)" % (2 + sample, n, depth, state_gb)}
+    out["reference_algorithm"] = reference_algorithm_baseline(n, depth)
+    return out
+
+
+def reference_algorithm_baseline(n_target, depth):
+    """The reference's own algorithm (simulation.rs:21-180: a hash map per gate over every basis
+    state, first insert then add, ascending inputs), restated by the oracle (orc_simulate_hashmap),
+    single-threaded as the reference is (README.md:96). Timed per gate on the first gates of the
+    brickwork at 10..22 qubits (a few seconds each); log2(seconds per gate) is fitted linearly in
+    the qubit count over 16..22 and extrapolated to n_target, labelled as such. The restatement
+    keys its maps by the 64-bit basis index; the reference keys them by n-element Vec<Qubit>
+    hashed with SipHash, so the reference itself is slower than this line."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import helpers as H  # oracle binding (baseline only)
+
+    rows = []
+    for n in range(10, 23, 2):
+        gates = W.brickwork(n, depth=depth, seed=1234)
+        k = 64 if n <= 16 else (24 if n <= 20 else 8)
+        prog = W.to_program(gates[:k])
+        t0 = time.perf_counter()
+        H.oracle_simulate(n, prog, method="hashmap")
+        t = (time.perf_counter() - t0) / k
+        rows.append({"qubits": n, "s_per_gate": t, "gates_timed": k, "gates_per_s": 1.0 / t})
+        log("bench: reference algorithm (hash map, 1 thread) %d qubits: %.4f s per gate" % (n, t))
+    xs = np.array([r["qubits"] for r in rows if r["qubits"] >= 16], dtype=float)
+    ys = np.log2([r["s_per_gate"] for r in rows if r["qubits"] >= 16])
+    slope, icpt = np.polyfit(xs, ys, 1)
+    t30 = 2.0 ** (slope * n_target + icpt)
+    g30 = len(W.brickwork(n_target, depth=depth, seed=1234))
+    return {"cores": 1, "unit": "gates/s", "kind": "port",
+            "algorithm": "per-gate hash map over all basis states (simulation.rs:21-180), oracle restatement",
+            "measured": rows, "fit_log2_s_per_gate_per_qubit": round(float(slope), 4),
+            "extrapolated": True,
+            "extrapolated_gates_per_s": 1.0 / t30,
+            "extrapolated_hours_per_step": t30 * g30 / 3600.0,
+            "note": "extrapolated from 16..22 qubits to %d; a direct run needs about 100 GB of hash maps"
+                    % n_target}
+
+
+def kernel_legs(args):
+    """Each standalone kernel on a 30-qubit state (complex128 unless noted), timed with HIP events
+    on the state's stream (qr_profile), for per-kernel HBM GB/s: reference-order gates (1, 2 and 3
+    qubits), dense and diagonal 3-qubit custom tables, the sampler passes, the single-lane fold
+    checker (24 qubits), norm, expectation, the streaming ceiling; and the per-op adjoint path at
+    24 qubits (2x2 and diagonal sweeps, gradient partials)."""
+    n = args.qubits
+    out = {}
+
+    def record(st, label):
+        for name, e in st.profile_read().items():
+            key = "%s/%s" % (label, name)
+            out[key] = {"launches": e["launches"], "ms_per_launch": round(e["total_ms"] / max(e["launches"], 1), 4),
+                        "gbs": round(e["bytes"] / max(e["total_ms"], 1e-9) / 1e6, 1)}
+            log("bench kernels: %-40s %4d launches %9.3f ms/launch %8.1f GB/s"
+                % (key, e["launches"], out[key]["ms_per_launch"], out[key]["gbs"]))
+
+    st = _capi.State(n)
+    st.set_options(fused=0)
+    u3 = W.random_unitary(3, 3)
+    d3 = np.diag(np.exp(1j * np.linspace(0.1, 2.0, 8)))
+    prog = W.to_program([("H", n - 1, (), 0.0, 0), ("Rx", 0, (), 0.7, 0), ("CNot", 1, (0,), 0.0, 0),
+                         ("Toffoli", 15, (2, 28), 0.0, 0)])
+    st.apply(prog)
+    st.profile_enable(True)
+    for _ in range(3):
+        st.apply(prog)
+    record(st, "reference_order")
+    st.profile_enable(False)
+    st.set_options(fused=1)
+    cust = W.to_program([("Custom", 20, (5, 25), 0.0, 0), ("Custom", 9, (12, 27), 0.0, 0)],
+                        {0: (np.ascontiguousarray(u3.T), np.ones(8, np.uint8)),
+                         1: (np.ascontiguousarray(d3.T), np.ones(8, np.uint8))})
+    st.apply(cust)
+    st.profile_enable(True)
+    for _ in range(3):
+        st.apply(cust)
+    u = np.array(W.uniforms(500, 99))
+    st.sample(u)
+    st.norm_sqr()
+    st.expectation(_capi.Observable(W.hea_observable(n)))
+    st.measure_stream(3)
+    record(st, "standalone")
+    st.close()
+    s24 = _capi.State(24)
+    s24.apply(W.to_program(W.brickwork(24, depth=4, seed=5)))
+    s24.set_options(exact_sample=1)
+    s24.profile_enable(True)
+    s24.sample(u)
+    record(s24, "checker_24q")
+    s24.close()
+    gates, _ = W.hea(24, 2, seed=7)
+    g = _capi.Grad(24)
+    g.set_options(jit=0)
+    g.forward(W.to_program(gates), _capi.Observable(W.hea_observable(24)))
+    g.backward()
+    return out
 
 
 def main():
@@ -417,6 +518,8 @@ def main():
                    "qubits": r["n"], "depth": args.depth, "gates": r["gates"], "precision": args.precision},
         "gates_per_s": round(r["gates"] / (ms * 1e-3), 1),
         "sweeps_per_step": r["plan"]["sweeps"],
+        "max_stage_gates": r["plan"]["max_stage_gates"],
+        "restore_sweeps": r["plan"]["restore_sweeps"],
         "tile_sweeps_per_step": r["plan"]["tile_sweeps"],
         "algorithmic_bytes_per_step": r["step_bytes"],
         "norm_error": r["norm_err"],
@@ -424,7 +527,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "qr_stage (compiled fused tile sweep)",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch_bytes},
+                     "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch_bytes,
+                     "stream_ceiling": {
+                         "kernel": "k_stream_scale (in-place read+write of the same state, one amplitude "
+                                   "per thread)",
+                         "ms": round(r["stream_ms"], 4),
+                         "gbs": round(per_launch_bytes / (r["stream_ms"] * 1e-3) / 1e9, 1),
+                         "frac_of_peak": round(per_launch_bytes / (r["stream_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "sweep_vs_stream": round(r["stream_ms"] / avg_ms, 4) if avg_ms > 0 else None}},
     }
     if not args.no_adjoint:
         out["adjoint"] = bench_adjoint(args)
@@ -434,6 +544,8 @@ def main():
         out["precision"] = precision_report(args, ms)
     if args.capacity:
         out["capacity"] = capacity(args)
+    if args.kernels:
+        out["kernels"] = kernel_legs(args)
     if not args.no_cpu_baseline:
         log("bench: CPU baseline")
         out["cpu_baseline"] = cpu_baseline(r["n"], args.depth)

@@ -298,6 +298,11 @@ typedef struct qr_profile_entry {
   double bytes; /* algorithmic HBM bytes moved by those launches */
 } qr_profile_entry;
 qr_status qr_profile_enable(qr_state* state, int enable);
+/* The streaming ceiling of a sweep of this state: `reps` launches of an in-place kernel that
+ * reads every amplitude, multiplies it by 1 and writes it back (one amplitude per thread, the
+ * fastest in-place shape measured); ms_out = average milliseconds per launch. The state's
+ * values are unchanged. No reference counterpart. */
+qr_status qr_measure_stream(qr_state* state, uint32_t reps, double* ms_out);
 qr_status qr_profile_read(qr_state* state, qr_profile_entry* entries, uint64_t capacity,
                           uint64_t* count_out);
 /* Statistics of the last qr_apply_program: sweeps launched and algorithmic bytes. */

@@ -103,6 +103,7 @@ _SIGNATURES = [
     ("qr_profile_read", C.c_int, [C.c_void_p, C.POINTER(QrProfileEntry), C.c_uint64,
                                   C.POINTER(C.c_uint64)]),
     ("qr_state_last_plan", C.c_int, [C.c_void_p, C.POINTER(QrPlanStats)]),
+    ("qr_measure_stream", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(C.c_double)]),
 ]
 EXPORTED = [s[0] for s in _SIGNATURES]
 
@@ -310,6 +311,12 @@ class State:
         check(lib().qr_state_last_plan(self._h, C.byref(s)))
         return {"gates": s.gates, "sweeps": s.sweeps, "tile_sweeps": s.tile_sweeps, "bytes": s.bytes,
                 "max_stage_gates": s.max_stage_gates, "restore_sweeps": s.restore_sweeps}
+
+    def measure_stream(self, reps=5):
+        """Average ms of an in-place read+write sweep of this state (the streaming ceiling)."""
+        v = C.c_double()
+        check(lib().qr_measure_stream(self._h, reps, C.byref(v)))
+        return v.value
 
     def profile_enable(self, on=True):
         check(lib().qr_profile_enable(self._h, 1 if on else 0))

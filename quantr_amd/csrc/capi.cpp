@@ -444,6 +444,30 @@ qr_status qr_batch_grad(uint64_t ninstances, const uint32_t* num_qubits, const q
   });
 }
 
+qr_status qr_measure_stream(qr_state* st, uint32_t reps, double* ms_out) {
+  return guard([&] {
+    if (!st || !ms_out) throw Error{QR_ERR_ARG, "NULL argument"};
+    if (reps == 0) throw Error{QR_ERR_ARG, "reps must be at least 1"};
+    hip_check(launch_stream_scale(st->prec, st->amp, st->dim, 1.0, st->stream), "stream kernel");  // warm-up
+    hipEvent_t a, b;
+    hip_check(hipEventCreate(&a), "hipEventCreate");
+    hip_check(hipEventCreate(&b), "hipEventCreate");
+    hipError_t e = hipEventRecord(a, st->stream);
+    for (uint32_t r = 0; r < reps && e == hipSuccess; ++r) {
+      ProfScope ps_(st, "stream_scale", 2.0 * (double)st->dim * st->bytes_per_amp());
+      e = launch_stream_scale(st->prec, st->amp, st->dim, 1.0, st->stream);
+    }
+    if (e == hipSuccess) e = hipEventRecord(b, st->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(b);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, b);
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    hip_check(e, "stream timing");
+    *ms_out = (double)ms / reps;
+  });
+}
+
 qr_status qr_profile_enable(qr_state* st, int enable) {
   return guard([&] {
     if (!st) throw Error{QR_ERR_ARG, "state is NULL"};

@@ -234,6 +234,37 @@ __global__ void k_densek(typename CT<R>::T* __restrict__ psi, typename CT<R>::T*
   }
 }
 
+// Dense 2^K x 2^K on K <= 4 target bits (+ controls) with the group in registers: one thread
+// per group of 2^K amplitudes, which it reads once, transforms and writes back in place.
+// off[j] is the offset of local basis state j (MSB-first local index).
+template <typename R, int K>
+__global__ __launch_bounds__(256) void k_densek_reg(typename CT<R>::T* __restrict__ psi, uint64_t ngroups, DenseKArgs a,
+                                                    const double2* __restrict__ M, const uint64_t* __restrict__ off) {
+  using C = typename CT<R>::T;
+  constexpr int D = 1 << K;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < ngroups; g += stride) {
+    const uint64_t base = insert_zero_bits(g, a.sorted_bits, K);
+    if ((base & a.cmask) != a.cmask) continue;
+    C v[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) v[j] = psi[base + off[j]];
+#pragma unroll
+    for (int r = 0; r < D; ++r) {
+      R ar = 0, ai = 0;
+#pragma unroll
+      for (int j = 0; j < D; ++j) {
+        const double2 m = M[r * D + j];
+        cmul_acc(ar, ai, (R)m.x, (R)m.y, (R)v[j].x, (R)v[j].y);
+      }
+      C o;
+      o.x = ar;
+      o.y = ai;
+      psi[base + off[r]] = o;
+    }
+  }
+}
+
 // 2^k diagonal phases over k bits (+ controls).
 template <typename R>
 __global__ void k_diagk(typename CT<R>::T* __restrict__ psi, uint64_t dim, DiagKArgs a,
@@ -625,6 +656,20 @@ __global__ __launch_bounds__(256) void k_dot_re_partial(const typename CT<R>::T*
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// The streaming ceiling of a sweep: every amplitude read, multiplied by a real factor and
+// written back in place, one amplitude per thread in index order (the fastest in-place shape
+// measured on MI355X).
+template <typename R>
+__global__ __launch_bounds__(256) void k_stream_scale(typename CT<R>::T* __restrict__ psi, uint64_t dim, R f) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < dim) {
+    typename CT<R>::T v = psi[i];
+    v.x *= f;
+    v.y *= f;
+    psi[i] = v;
+  }
+}
+
 __global__ void k_stamp(uint64_t* out) {
   if (threadIdx.x == 0 && blockIdx.x == 0) out[0] = wall_clock64();
 }
@@ -740,6 +785,27 @@ hipError_t launch_diag1(int prec, void* psi, uint64_t dim, int t, uint64_t cmask
   return hipGetLastError();
 }
 
+hipError_t launch_densek_reg(int prec, void* psi, uint64_t ngroups, const DenseKArgs& a, const double2* M,
+                             const uint64_t* off, hipStream_t s) {
+  const unsigned g = grid_for(ngroups, 256, 1u << 18);
+#define QR_DENSEK_CASE(KK)                                                                     \
+  case KK:                                                                                     \
+    if (prec == 0)                                                                             \
+      dev::k_densek_reg<double, KK><<<g, 256, 0, s>>>((double2*)psi, ngroups, a, M, off);      \
+    else                                                                                       \
+      dev::k_densek_reg<float, KK><<<g, 256, 0, s>>>((float2*)psi, ngroups, a, M, off);        \
+    break;
+  switch (a.k) {
+    QR_DENSEK_CASE(2)
+    QR_DENSEK_CASE(3)
+    QR_DENSEK_CASE(4)
+    default:
+      return hipErrorInvalidValue;
+  }
+#undef QR_DENSEK_CASE
+  return hipGetLastError();
+}
+
 hipError_t launch_densek(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
                          const DenseKArgs& a, const double2* M, const uint64_t* off, hipStream_t s) {
   const int block = tmp_groups >= 256 ? 256 : (int)tmp_groups;
@@ -844,6 +910,16 @@ hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_
     dev::k_dot_re_partial<double><<<nblocks, 256, 0, s>>>((const double2*)a, (const double2*)b, dim, part);
   else
     dev::k_dot_re_partial<float><<<nblocks, 256, 0, s>>>((const float2*)a, (const float2*)b, dim, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_stream_scale(int prec, void* psi, uint64_t dim, double f, hipStream_t s) {
+  const uint64_t g = (dim + 255) / 256;
+  if (g > 0x7fffffffull) return hipErrorInvalidValue;
+  if (prec == 0)
+    dev::k_stream_scale<double><<<(unsigned)g, 256, 0, s>>>((double2*)psi, dim, f);
+  else
+    dev::k_stream_scale<float><<<(unsigned)g, 256, 0, s>>>((float2*)psi, dim, (float)f);
   return hipGetLastError();
 }
 

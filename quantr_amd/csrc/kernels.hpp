@@ -93,6 +93,9 @@ hipError_t launch_diag1(int prec, void* psi, uint64_t dim, int t, uint64_t cmask
                         hipStream_t s);
 hipError_t launch_densek(int prec, void* psi, void* tmp, uint64_t tmp_groups, uint64_t ngroups,
                          const DenseKArgs& a, const double2* M, const uint64_t* off, hipStream_t s);
+// Dense 2^k x 2^k, 2 <= k <= 4, each group of amplitudes in registers (no staging).
+hipError_t launch_densek_reg(int prec, void* psi, uint64_t ngroups, const DenseKArgs& a, const double2* M,
+                             const uint64_t* off, hipStream_t s);
 hipError_t launch_diagk(int prec, void* psi, uint64_t dim, const DiagKArgs& a, const double2* D,
                         hipStream_t s);
 hipError_t launch_tile(int prec, int K, int RB, bool prefetch, void* psi, const TileArgs& a, hipStream_t s);
@@ -130,6 +133,8 @@ hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const Pa
 // part[b] = partial sums of Re sum_j conj(a_j) b_j over nblocks blocks.
 hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
                                  hipStream_t s);
+// psi[i] *= f for every amplitude, in place (the streaming ceiling of a sweep).
+hipError_t launch_stream_scale(int prec, void* psi, uint64_t dim, double f, hipStream_t s);
 // out[0] = the device wall clock (constant-rate counter).
 hipError_t launch_stamp(uint64_t* out, hipStream_t s);
 // out[r] = sum of part[r * ncols + 0 .. ncols-1] (fixed order), r < nrows.

@@ -172,6 +172,12 @@ void launch_standalone_raw(int prec, uint64_t dim, void* amp, DevBuf& tmp, const
       for (int q = 0; q < op.k; ++q) a.sorted_bits[q] = (uint8_t)sb[q];
       const uint64_t dimk = 1ull << op.k;
       const uint64_t ngroups = dim >> op.k;
+      if (op.k >= 2 && op.k <= 4) {  // the group fits in registers
+        hip_check(launch_densek_reg(prec, amp, ngroups, a, (const double2*)(dblob + l.coefs),
+                                    (const uint64_t*)(dblob + l.off), s),
+                  "densek");
+        break;
+      }
       uint64_t tg = std::min<uint64_t>(ngroups, std::max<uint64_t>(256, (64ull << 20) / (dimk * 16)));
       tg = std::max<uint64_t>(tg / 256, 1) * 256;
       if (ngroups < 256) tg = ngroups;
