@@ -92,6 +92,21 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
       if ((m >> j) & 1) x |= 1ull << gbit(mi, NT + j);
     return x;
   };
+  // Amplitudes move between memory and registers one component at a time: a whole-struct
+  // copy can be lowered to a memcpy through a private stack slot (scratch memory) in stages
+  // without gates, while the component copies still combine into one 16/8-byte access.
+  auto load_amp = [&](const std::string& dst, const std::string& arr, const char* idx, uint64_t off) {
+    std::ostringstream e;
+    e << "{ const " << C << "* s_ = &" << arr << "[" << idx << " | " << hex64(off) << "]; " << dst << ".x = s_->x; "
+      << dst << ".y = s_->y; }";
+    return e.str();
+  };
+  auto store_amp = [&](const std::string& src, const std::string& arr, const char* idx, uint64_t off) {
+    std::ostringstream e;
+    e << "{ " << C << "* d_ = &" << arr << "[" << idx << " | " << hex64(off) << "]; d_->x = " << src << ".x; d_->y = "
+      << src << ".y; }";
+    return e.str();
+  };
   std::ostringstream o;
   // occupancy target of the stage kernels: QR_STAGE_WAVES_PER_EU (1-8) asks the compiler to
   // fit that many wavefronts per SIMD (fewer registers per thread); unset leaves it free
@@ -124,7 +139,7 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   o << "  uint64_t g0 = " << thread_part(mi) << ";\n";
   for (size_t v = 0; v < sets.size(); ++v)
     for (int m = 0; m < NR; ++m)
-      o << "  " << sets[v] << ".v[" << m << "] = " << arrays[v] << "[g0 | " << hex64(reg_off(mi, m)) << "];\n";
+      o << "  " << load_amp(sets[v] + ".v[" + std::to_string(m) + "]", arrays[v], "g0", reg_off(mi, m)) << "\n";
   if (persistent) {
     o << "  qr::dev::Regs<R, " << RB << "> nx;\n";
     o << "  while (true) {\n";
@@ -133,7 +148,8 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
     o << "  uint64_t nbase = 0;\n";
     o << "  if (more) {\n    nbase = qr::dev::deposit_bits(ntile, nontile);\n";
     o << "    const uint64_t ng = nbase" << thread0 << ";\n";
-    for (int m = 0; m < NR; ++m) o << "    nx.v[" << m << "] = psi[ng | " << hex64(reg_off(0, m)) << "];\n";
+    for (int m = 0; m < NR; ++m)
+      o << "    " << load_amp("nx.v[" + std::to_string(m) + "]", "psi", "ng", reg_off(0, m)) << "\n";
     o << "  }\n";
   }
   auto coef = [&](size_t i) {
@@ -324,12 +340,12 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   }
   for (size_t v = 0; v < sets.size(); ++v)
     for (int m = 0; m < NR; ++m)
-      o << "  " << arrays[v] << "[g0 | " << hex64(reg_off(mi, m)) << "] = " << sets[v] << ".v[" << m << "];\n";
+      o << "  " << store_amp(sets[v] + ".v[" + std::to_string(m) + "]", arrays[v], "g0", reg_off(mi, m)) << "\n";
   if (persistent) {
     o << "  if (!more) break;\n";
     o << "  tile = ntile;\n  base = nbase;\n";
     o << "  g0 = " << thread_part(0) << ";\n";
-    for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "] = nx.v[" << m << "];\n";
+    for (int m = 0; m < NR; ++m) o << "  r.v[" << m << "].x = nx.v[" << m << "].x; r.v[" << m << "].y = nx.v[" << m << "].y;\n";
     o << "  }\n";
   }
   o << "}\n";
@@ -534,6 +550,33 @@ void jit_prepare(const std::vector<std::string>& sources) {
   std::lock_guard<std::mutex> lk(g_mu);
   for (size_t i = 0; i < todo.size(); ++i)
     if (!g_cache.count(todo[i])) g_cache[todo[i]] = made[i];
+}
+
+// The occupancy hint asks the compiler to fit more wavefronts per SIMD; for a stage whose
+// code then needs more registers than that allows, the compiler spills to scratch memory,
+// which costs HBM traffic. Such stages are compiled again without the hint.
+void jit_prepare_without_spills(std::vector<std::string>& sources) {
+  jit_prepare(sources);
+  std::vector<size_t> redo;
+  std::vector<std::string> plain;
+  for (size_t i = 0; i < sources.size(); ++i) {
+    if (sources[i].empty()) continue;
+    const size_t at = sources[i].find(" __attribute__((amdgpu_waves_per_eu(");
+    if (at == std::string::npos) continue;
+    int scratch = 0;
+    if (hipFuncGetAttribute(&scratch, HIP_FUNC_ATTRIBUTE_LOCAL_SIZE_BYTES, jit_function(sources[i])) != hipSuccess ||
+        scratch <= 0)
+      continue;
+    const size_t end = sources[i].find(")))", at);
+    if (end == std::string::npos) continue;
+    std::string p = sources[i];
+    p.erase(at, end + 3 - at);
+    redo.push_back(i);
+    plain.push_back(std::move(p));
+  }
+  if (redo.empty()) return;
+  jit_prepare(plain);
+  for (size_t j = 0; j < redo.size(); ++j) sources[redo[j]] = std::move(plain[j]);
 }
 
 hipFunction_t jit_function(const std::string& source) {

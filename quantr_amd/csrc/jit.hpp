@@ -16,6 +16,9 @@ std::vector<std::string> jit_stage_sources(const std::vector<Stage>& stages, int
                                            bool dual = false);
 // Compiles (in parallel) and loads every source not yet in the process-wide cache. Throws Error.
 void jit_prepare(const std::vector<std::string>& sources);
+// jit_prepare, then stages that spill registers under their occupancy hint are compiled again
+// without it; `sources` then names the kernels to use.
+void jit_prepare_without_spills(std::vector<std::string>& sources);
 // The kernel of a prepared source. Throws Error if it was not prepared.
 hipFunction_t jit_function(const std::string& source);
 // Launches a prepared stage: one workgroup per tile.

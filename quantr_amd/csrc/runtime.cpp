@@ -268,7 +268,7 @@ void prepare_compiled(CachedPlan& cp, int prec, int RB, bool persistent, bool du
   std::lock_guard<std::mutex> lk(cp.compile_mu);
   if (!cp.fns.empty() || cp.stages.empty()) return;
   cp.src = jit_stage_sources(cp.stages, prec, RB, persistent, dual);
-  jit_prepare(cp.src);
+  jit_prepare_without_spills(cp.src);
   std::vector<hipFunction_t> f(cp.stages.size(), nullptr);
   for (size_t i = 0; i < cp.stages.size(); ++i)
     if (cp.stages[i].tile) f[i] = jit_function(cp.src[i]);
