@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--reg-bits", type=int, default=0)
     ap.add_argument("--stage-ops", type=int, default=0, help="gates per tile sweep at most (0 = auto)")
     ap.add_argument("--jit", type=int, default=2)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="plans by estimate only (no measured selection of plans and tile orders)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-qubits", type=int, default=20)
@@ -83,9 +85,27 @@ def bench_simulate(args):
     gates = W.brickwork(n, depth=args.depth, seed=1234)
     prog = W.to_program(gates).c_struct()
     st = _capi.State(n, prec)
+    tune = 0 if args.no_tune else 1
     st.set_options(fused=1, tile_bits=args.tile_bits, low_bits=args.low_bits, tile_variant=args.tile_variant,
-                   reg_bits=args.reg_bits, jit=args.jit, stage_ops=args.stage_ops)
+                   reg_bits=args.reg_bits, jit=args.jit, stage_ops=args.stage_ops, tune=tune)
     log("bench: %d-qubit %s brickwork depth %d, %d gates" % (n, args.precision, args.depth, len(gates)))
+    # measured plans: the first applications of the structure time candidate plans and tile
+    # orders (each one a full application of the circuit); they run before the warmup and the
+    # clock, like planning and compilation
+    tuning = {"enabled": bool(tune), "applications": 0, "seconds": 0.0, "runs": []}
+    t_tune = time.perf_counter()
+    while tune and tuning["applications"] < 16:
+        st.reset()
+        st.apply(prog)
+        st.synchronize()
+        rec = st.last_plan()
+        if not rec["tuning"]:
+            break
+        tuning["applications"] += 1
+        tuning["runs"].append({"sweeps": rec["sweeps"], "xcd_order_sweeps": rec["xcd_order_sweeps"]})
+    tuning["seconds"] = round(time.perf_counter() - t_tune, 2)
+    if tune:
+        log("bench: tuning took %d measured applications, %.2f s" % (tuning["applications"], tuning["seconds"]))
     for i in range(args.warmup):
         t0 = time.perf_counter()
         st.reset()
@@ -132,7 +152,7 @@ def bench_simulate(args):
     step_bytes = plan["bytes"] + init_bytes
     log("bench: %.3f ms/step, %d sweeps (%d tile), norm-1 = %.2e" % (ms, plan["sweeps"], plan["tile_sweeps"], norm - 1))
     return {"ms": ms, "plan": plan, "prof": prof, "step_bytes": step_bytes, "gates": len(gates), "n": n,
-            "B": B, "norm_err": abs(norm - 1.0), "sampling": sampling, "stream_ms": stream_ms}
+            "B": B, "norm_err": abs(norm - 1.0), "sampling": sampling, "stream_ms": stream_ms, "tuning": tuning}
 
 
 def bench_adjoint(args):
@@ -520,6 +540,8 @@ def main():
         "sweeps_per_step": r["plan"]["sweeps"],
         "max_stage_gates": r["plan"]["max_stage_gates"],
         "restore_sweeps": r["plan"]["restore_sweeps"],
+        "xcd_order_sweeps": r["plan"]["xcd_order_sweeps"],
+        "tuning": r["tuning"],
         "tile_sweeps_per_step": r["plan"]["tile_sweeps"],
         "algorithmic_bytes_per_step": r["step_bytes"],
         "norm_error": r["norm_err"],

@@ -894,7 +894,7 @@ class Circuit {
   std::optional<states::SuperPosition> register_;
   bool config_progress_ = false;
   qr_precision precision_ = QR_C128;
-  qr_options options_{1, 0, 0, 2, 0, 0, 2, 0};
+  qr_options options_{1, 0, 0, 2, 0, 0, 2, 0, 0};
   bool have_options_ = false;
 };
 

@@ -28,8 +28,9 @@ extern "C" {
 #endif
 
 /* 2: qr_options gained stage_ops (a struct of 8 words; version 1 had 7).
- * 3: qr_plan_stats gained max_stage_gates and restore_sweeps. */
-#define QR_ABI_VERSION 3u
+ * 3: qr_plan_stats gained max_stage_gates and restore_sweeps.
+ * 4: qr_options gained tune (9 words); qr_plan_stats gained tuning and xcd_order_sweeps. */
+#define QR_ABI_VERSION 4u
 
 typedef enum qr_status {
   QR_OK = 0,
@@ -156,6 +157,14 @@ typedef struct qr_options {
   uint32_t stage_ops;    /* gates fused into one tile sweep at most (0 = auto). A capped
                             stage leaves its tail to later sweeps, which balances ALU work
                             between sweeps; results do not depend on it.                 */
+  uint32_t tune;         /* 0 (default): plans chosen by estimate, identical on every call.
+                            1: measured plans. The first applications of a circuit
+                            structure time candidate plans (the planner's best estimates)
+                            and, per tile sweep, two orders of dealing tiles to workgroups;
+                            later applications use the fastest. Every application computes
+                            the circuit (candidates agree within the fused mode's
+                            tolerance); qr_plan_stats.tuning marks the measuring ones.
+                            Needs jit != 0 and tile_variant != 2; ignored otherwise.     */
 } qr_options;
 
 typedef struct qr_state qr_state;
@@ -314,6 +323,10 @@ typedef struct qr_plan_stats {
   uint64_t max_stage_gates; /* most gates fused into one tile sweep        */
   uint64_t restore_sweeps;  /* gate-free tile sweeps that restore the
                                state's bit order (moving layout)           */
+  uint64_t tuning;          /* 1 if this application measured a candidate
+                               (qr_options.tune), else 0                    */
+  uint64_t xcd_order_sweeps;/* tile sweeps whose tiles were dealt to the
+                               XCDs in contiguous eighths                  */
 } qr_plan_stats;
 qr_status qr_state_last_plan(const qr_state* state, qr_plan_stats* out);
 

@@ -44,7 +44,7 @@ class QrObservable(C.Structure):
 class QrOptions(C.Structure):
     _fields_ = [("fused", C.c_uint32), ("tile_bits", C.c_uint32), ("low_bits", C.c_uint32),
                 ("exact_sample", C.c_uint32), ("tile_variant", C.c_uint32), ("reg_bits", C.c_uint32),
-                ("jit", C.c_uint32), ("stage_ops", C.c_uint32)]
+                ("jit", C.c_uint32), ("stage_ops", C.c_uint32), ("tune", C.c_uint32)]
 
 
 class QrBatchOptions(C.Structure):
@@ -54,7 +54,8 @@ class QrBatchOptions(C.Structure):
 
 class QrPlanStats(C.Structure):
     _fields_ = [("gates", C.c_uint64), ("sweeps", C.c_uint64), ("tile_sweeps", C.c_uint64),
-                ("bytes", C.c_double), ("max_stage_gates", C.c_uint64), ("restore_sweeps", C.c_uint64)]
+                ("bytes", C.c_double), ("max_stage_gates", C.c_uint64), ("restore_sweeps", C.c_uint64),
+                ("tuning", C.c_uint64), ("xcd_order_sweeps", C.c_uint64)]
 
 
 class QrProfileEntry(C.Structure):
@@ -108,7 +109,7 @@ _SIGNATURES = [
 EXPORTED = [s[0] for s in _SIGNATURES]
 
 _lib = None
-ABI_VERSION = 3  # QR_ABI_VERSION of include/quantr_hip.h; the structures above follow it
+ABI_VERSION = 4  # QR_ABI_VERSION of include/quantr_hip.h; the structures above follow it
 
 
 class QrError(RuntimeError):
@@ -261,8 +262,8 @@ class State:
         return self._h
 
     def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2,
-                    stage_ops=0):
-        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops)
+                    stage_ops=0, tune=0):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops, tune)
         check(lib().qr_state_set_options(self._h, C.byref(o)))
 
     def reset(self):
@@ -310,7 +311,8 @@ class State:
         s = QrPlanStats()
         check(lib().qr_state_last_plan(self._h, C.byref(s)))
         return {"gates": s.gates, "sweeps": s.sweeps, "tile_sweeps": s.tile_sweeps, "bytes": s.bytes,
-                "max_stage_gates": s.max_stage_gates, "restore_sweeps": s.restore_sweeps}
+                "max_stage_gates": s.max_stage_gates, "restore_sweeps": s.restore_sweeps, "tuning": s.tuning,
+                "xcd_order_sweeps": s.xcd_order_sweeps}
 
     def measure_stream(self, reps=5):
         """Average ms of an in-place read+write sweep of this state (the streaming ceiling)."""
@@ -357,8 +359,8 @@ class Grad:
         return v.value
 
     def set_options(self, fused=1, tile_bits=0, low_bits=0, exact_sample=2, tile_variant=0, reg_bits=0, jit=2,
-                    stage_ops=0):
-        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops)
+                    stage_ops=0, tune=0):
+        o = QrOptions(fused, tile_bits, low_bits, exact_sample, tile_variant, reg_bits, jit, stage_ops, tune)
         check(lib().qr_grad_set_options(self._h, C.byref(o)))
 
     def num_params(self):

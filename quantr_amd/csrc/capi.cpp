@@ -171,6 +171,7 @@ qr_status qr_state_set_options(qr_state* st, const qr_options* o) {
     if (o->reg_bits != 0 && (o->reg_bits < 3 || o->reg_bits > 5)) throw Error{QR_ERR_ARG, "reg_bits must be 0, 3, 4 or 5"};
     if (o->jit > 2) throw Error{QR_ERR_ARG, "jit must be 0, 1 or 2"};
     if (o->stage_ops > 65536) throw Error{QR_ERR_ARG, "stage_ops must be <= 65536"};
+    if (o->tune > 1) throw Error{QR_ERR_ARG, "tune must be 0 or 1"};
     st->opt = *o;
   });
 }

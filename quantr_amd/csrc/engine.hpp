@@ -207,6 +207,11 @@ constexpr uint32_t kBalanceMinQubits = 25;
 // plan_fused itself.
 std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
                                        int bytes_per_amp, bool dual);
+// The balanced planner's distinct candidate plans (at most `count`), lowest estimate first;
+// one plan (plan_fused's) where plan_fused_balanced would not balance.
+std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vector<FastOp>& ops,
+                                                     const PlanParams& pp, int bytes_per_amp, bool dual,
+                                                     int count);
 // Appends what the planner's output depends on for one op (kind, bits, controls and the
 // value-derived forms), so that programs with equal keys share one plan; coefficients are
 // then taken from each program's own ops (TileOpHost::src).

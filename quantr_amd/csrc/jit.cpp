@@ -126,12 +126,17 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   if (dual) o << C << "* __restrict__ lam, ";
   o << "const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles";
   if (dual) o << ", double* __restrict__ part, uint64_t nslots";
+  else o << ", uint32_t order";
   o << ") {\n";
   o << "  using R = " << R << ";\n";
   o << "  __shared__ " << C << " lds[" << (split ? (1 << (K - 1)) : (lds2 ? 2 : 1) * (1 << K)) << "];\n";
   o << "  const uint32_t tid = threadIdx.x;\n";
   o << "  uint64_t tile = blockIdx.x;\n";
   o << "  if (tile >= ntiles) return;\n";
+  // order 1 deals tiles so that workgroup w takes tile (w mod 8) * ntiles/8 + w / 8: with
+  // workgroups handed to the 8 XCDs in turn, each XCD walks one contiguous eighth of the tiles
+  if (!dual && !persistent)
+    o << "  if (order == 1u && (ntiles & 7ull) == 0) tile = (tile & 7ull) * (ntiles >> 3) + (tile >> 3);\n";
   o << "  uint64_t base = qr::dev::deposit_bits(tile, nontile);\n";
   for (const auto& v : sets) o << "  qr::dev::Regs<R, " << RB << "> " << v << ";\n";
   int mi = 0;
@@ -587,7 +592,7 @@ hipFunction_t jit_function(const std::string& source) {
 }
 
 hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* psi, const double2* coefs,
-                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s) {
+                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s, uint32_t order) {
   if (!fn) return hipErrorInvalidValue;
   if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
   unsigned grid = (unsigned)ntiles;
@@ -601,7 +606,8 @@ hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* ps
     if (cus <= 0) cus = 256;
     grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
   }
-  void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles};
+  if (persistent) order = 0;
+  void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles, &order};
   return hipModuleLaunchKernel(fn, grid, 1, 1, threads, 1, 1, 0, s, args, nullptr);
 }
 

@@ -997,10 +997,14 @@ double plan_cost(const std::vector<Stage>& st, int bytes_per_amp) {
 
 }  // namespace
 
-std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
-                                       int bytes_per_amp, bool dual) {
-  if (pp.max_ops > 0 || dual || n < kBalanceMinQubits || pp.tile_bits >= (int)n)
-    return plan_fused(n, ops, pp, bytes_per_amp, dual);
+std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vector<FastOp>& ops,
+                                                     const PlanParams& pp, int bytes_per_amp, bool dual,
+                                                     int count) {
+  std::vector<std::vector<Stage>> out;
+  if (pp.max_ops > 0 || dual || n < kBalanceMinQubits || pp.tile_bits >= (int)n) {
+    out.push_back(plan_fused(n, ops, pp, bytes_per_amp, dual));
+    return out;
+  }
   static const int caps[] = {0, 160, 140, 120, 100, 90, 80};
   constexpr int nc = (int)(sizeof(caps) / sizeof(caps[0]));
   std::vector<std::vector<Stage>> plans(nc);
@@ -1019,16 +1023,34 @@ std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& op
   for (auto& t : th) t.join();
   for (auto& e : errs)
     if (e) std::rethrow_exception(e);
-  int best = 0;
-  double best_cost = plan_cost(plans[0], bytes_per_amp);
-  for (int i = 1; i < nc; ++i) {
-    const double c = plan_cost(plans[i], bytes_per_amp);
-    if (c < best_cost) {
-      best_cost = c;
-      best = i;
-    }
+  // ascending estimate; the first of equal estimates (the lower cap index) first
+  std::vector<int> idx(nc);
+  std::vector<double> cost(nc);
+  for (int i = 0; i < nc; ++i) {
+    idx[i] = i;
+    cost[i] = plan_cost(plans[i], bytes_per_amp);
   }
-  return std::move(plans[best]);
+  std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) { return cost[a] < cost[b]; });
+  // caps that change nothing give the same plan: keep one of each (same gates per stage)
+  auto shape = [](const std::vector<Stage>& st) {
+    std::vector<int> g;
+    for (const Stage& s : st) g.push_back(s.tile ? s.ngates : -1 - s.standalone);
+    return g;
+  };
+  std::vector<std::vector<int>> seen;
+  for (int i : idx) {
+    if ((int)out.size() >= std::max(1, count)) break;
+    std::vector<int> sh = shape(plans[i]);
+    if (std::find(seen.begin(), seen.end(), sh) != seen.end()) continue;
+    seen.push_back(std::move(sh));
+    out.push_back(std::move(plans[i]));
+  }
+  return out;
+}
+
+std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
+                                       int bytes_per_amp, bool dual) {
+  return std::move(plan_fused_candidates(n, ops, pp, bytes_per_amp, dual, 1)[0]);
 }
 
 }  // namespace qr

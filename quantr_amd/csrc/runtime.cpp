@@ -275,43 +275,136 @@ void prepare_compiled(CachedPlan& cp, int prec, int RB, bool persistent, bool du
   cp.fns = std::move(f);
 }
 
+// Candidate plans a tuning structure measures (the planner's lowest estimates).
+constexpr int kTuneCandidates = 3;
+
+namespace {
+
+// Events around every launch of one measuring application, destroyed with it.
+struct StageTimer {
+  std::vector<hipEvent_t> ev;
+  explicit StageTimer(size_t stages) : ev(2 * stages, nullptr) {
+    for (auto& e : ev) hip_check(hipEventCreate(&e), "hipEventCreate");
+  }
+  ~StageTimer() {
+    for (auto& e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  StageTimer(const StageTimer&) = delete;
+  StageTimer& operator=(const StageTimer&) = delete;
+};
+
+// The tuning record of a structure, made on first use: candidate 0 is `cp` itself.
+std::shared_ptr<TuneState> tune_state(qr_state* st, const std::shared_ptr<CachedPlan>& cp,
+                                      const std::vector<FastOp>& ops, const PlanParams& pp) {
+  std::lock_guard<std::mutex> lk(cp->tune_mu);
+  if (cp->tune) return cp->tune;
+  auto ts = std::make_shared<TuneState>();
+  ts->cands.push_back(cp);
+  std::vector<std::vector<Stage>> alt =
+      plan_fused_candidates(st->n, ops, pp, (int)st->bytes_per_amp(), false, kTuneCandidates);
+  for (size_t i = 1; i < alt.size(); ++i) {
+    auto c = std::make_shared<CachedPlan>();
+    c->stages = std::move(alt[i]);
+    ts->cands.push_back(std::move(c));
+  }
+  ts->cand_ms.assign(ts->cands.size(), 0.0);
+  ts->stage_ms.resize(ts->cands.size());
+  cp->tune = ts;
+  return ts;
+}
+
+}  // namespace
+
 // Plans (fused tile sweeps, or one sweep per op), uploads and launches a fast-form program.
 void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_stats* stats) {
   const PlanParams pp = plan_params(st);
   const bool persistent = st->opt.tile_variant == 2;
   std::shared_ptr<CachedPlan> cp = get_plan(st, ops, pp, fused, false, persistent);
+  // ---- measured selection (qr_options.tune): pick this application's plan and tile orders
+  std::shared_ptr<TuneState> ts;
+  std::unique_lock<std::mutex> tune_lock;
+  int step = -1;  // >= 0: this application measures
+  const std::vector<uint32_t>* orders = nullptr;
+  std::shared_ptr<CachedPlan> use = cp;
+  if (fused && st->opt.tune && st->opt.jit != 0 && !persistent) {
+    ts = tune_state(st, cp, ops, pp);
+    tune_lock = std::unique_lock<std::mutex>(ts->mu);
+    const int C = (int)ts->cands.size();
+    if (ts->step <= C) {
+      step = ts->step;
+      use = ts->cands[step < C ? step : ts->chosen];
+    } else {
+      use = ts->cands[ts->chosen];
+      orders = &ts->order;
+      tune_lock.unlock();
+    }
+  }
   PackedPlan P;
-  P.stages = cp->stages;
+  P.stages = use->stages;
   std::vector<Stage>& stages = P.stages;
   pack_plan(st, ops, P);
   // ---- runtime-specialised stages: always with jit = 1, from 28 qubits or from the second
-  // application of a structure with jit = 2
+  // application of a structure with jit = 2, and for every measured plan
   bool use_jit = false;
   if (fused && st->opt.jit != 0) {
-    use_jit = st->opt.jit == 1 || st->n >= 28 || cp->runs > 0;
-    if (use_jit) prepare_compiled(*cp, st->prec, pp.reg_bits, persistent, false);
+    use_jit = st->opt.jit == 1 || st->n >= 28 || cp->runs > 0 || ts != nullptr;
+    if (use_jit) prepare_compiled(*use, st->prec, pp.reg_bits, persistent, false);
   }
   cp->runs++;
+  std::unique_ptr<StageTimer> timer;
+  if (step >= 0) timer = std::make_unique<StageTimer>(stages.size());
+  const uint32_t measure_order = step >= 0 && step == (int)ts->cands.size() ? 1u : 0u;
   // ---- launch --------------------------------------------------------------------------
   qr_plan_stats ps{};
   for (size_t si = 0; si < stages.size(); ++si) {
     const Stage& s = stages[si];
     ps.sweeps++;
     ps.bytes += s.bytes;
+    if (timer) hip_check(hipEventRecord(timer->ev[2 * si], st->stream), "hipEventRecord");
     if (s.tile) {
       ps.tile_sweeps++;
       const TileArgs a = tile_args(st, P, si);
       ProfScope ps_(st, "tile_sweep", s.bytes);
-      if (use_jit)
-        hip_check(jit_launch(cp->fns[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles, a.nontile_mask,
-                             st->stream),
+      if (use_jit) {
+        uint32_t order = step >= 0 ? measure_order : (orders ? (*orders)[si] : 0u);
+        if (persistent || (a.ntiles & 7) != 0) order = 0;
+        ps.xcd_order_sweeps += order;
+        hip_check(jit_launch(use->fns[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles,
+                             a.nontile_mask, st->stream, order),
                   "compiled tile kernel");
-      else
+      } else {
         hip_check(launch_tile(st->prec, s.k, pp.reg_bits, st->opt.tile_variant == 2, st->amp, a, st->stream),
                   "tile kernel");
-      continue;
+      }
+    } else {
+      launch_standalone(st, st->amp, ops[s.standalone], P, si);
     }
-    launch_standalone(st, st->amp, ops[s.standalone], P, si);
+    if (timer) hip_check(hipEventRecord(timer->ev[2 * si + 1], st->stream), "hipEventRecord");
+  }
+  // ---- a measuring application records its times and advances the selection
+  if (timer) {
+    hip_check(hipEventSynchronize(timer->ev.back()), "hipEventSynchronize");
+    std::vector<float> ms(stages.size(), 0.0f);
+    double total = 0.0;
+    for (size_t si = 0; si < stages.size(); ++si) {
+      hip_check(hipEventElapsedTime(&ms[si], timer->ev[2 * si], timer->ev[2 * si + 1]), "hipEventElapsedTime");
+      total += ms[si];
+    }
+    const int C = (int)ts->cands.size();
+    if (step < C) {
+      ts->cand_ms[step] = total;
+      ts->stage_ms[step] = ms;
+      if (step == C - 1)
+        ts->chosen = (int)(std::min_element(ts->cand_ms.begin(), ts->cand_ms.end()) - ts->cand_ms.begin());
+    } else {
+      const std::vector<float>& base = ts->stage_ms[ts->chosen];
+      ts->order.assign(stages.size(), 0u);
+      for (size_t si = 0; si < stages.size(); ++si)
+        if (stages[si].tile && ms[si] < base[si]) ts->order[si] = 1u;
+    }
+    ts->step++;
+    ps.tuning = 1;
   }
   ps.gates = 0;
   for (const Stage& s : stages) {

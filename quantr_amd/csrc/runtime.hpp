@@ -57,7 +57,7 @@ struct qr_state {
   void* amp = nullptr;
   hipStream_t stream = nullptr;
   int device = 0;
-  qr_options opt{1, 0, 0, 2, 0, 0, 2};
+  qr_options opt{1, 0, 0, 2, 0, 0, 2, 0, 0};
   qr::DevBuf aux;   // plan / coefficients / small uploads
   qr::DevBuf aux2;  // partial sums, sampler scratch
   qr::DevBuf tmp;   // wide-gate staging (bounded, never state-sized)
@@ -132,6 +132,8 @@ struct PackedPlan {
   uint8_t* dblob = nullptr;
 };
 
+struct TuneState;
+
 // A plan shared by every program of one structure, with its compiled stages.
 struct CachedPlan {
   std::vector<Stage> stages;
@@ -140,6 +142,22 @@ struct CachedPlan {
   std::vector<hipFunction_t> fns;  // their kernels
   std::atomic<uint64_t> runs{0};  // applications (states on several threads may share a plan)
   uint64_t stamp = 0;
+  std::mutex tune_mu;               // guards `tune`
+  std::shared_ptr<TuneState> tune;  // measured selection (qr_options.tune), made on first use
+};
+
+// Measured selection for one structure. Applications with qr_options.tune measure, in turn:
+// each candidate plan with tiles in index order (step 0..C-1), then the fastest candidate
+// with tiles dealt to the XCDs in contiguous eighths (step C). Each tile sweep then keeps the
+// faster of its two orders. `mu` serialises the measuring applications of the structure.
+struct TuneState {
+  std::mutex mu;
+  std::vector<std::shared_ptr<CachedPlan>> cands;  // [0] is the plan the estimate picks
+  std::vector<double> cand_ms;                     // per candidate, sum of its sweeps' times
+  std::vector<std::vector<float>> stage_ms;        // per candidate, per stage (index order)
+  int step = 0;                                    // next measurement; C + 1 = done
+  int chosen = 0;
+  std::vector<uint32_t> order;  // per stage of the chosen plan: 0 index order, 1 per-XCD
 };
 
 std::shared_ptr<CachedPlan> get_plan(const qr_state* st, const std::vector<FastOp>& ops, const PlanParams& pp,

@@ -29,7 +29,7 @@ def test_header_symbols_exported():
 
 def test_version():
     lib = _capi.lib()
-    assert lib.qr_abi_version() == _capi.ABI_VERSION == 3
+    assert lib.qr_abi_version() == _capi.ABI_VERSION == 4
     assert b"gfx950" in lib.qr_version()
 
 

@@ -285,6 +285,43 @@ def test_balanced_plan_25q_vs_exact():
     assert plan["sweeps"] >= len(gates) // 7
 
 
+def test_measured_plans_25q():
+    """tune = 1: the first applications of a structure measure candidate plans and tile
+    orders. Every one of them computes the circuit (within the fused tolerance of the
+    reference-order result); once measured, applications repeat bit for bit."""
+    n = 25
+    gates = W.brickwork(n, depth=40, seed=78)
+    p = W.to_program(gates)
+    say("measured plans 25q exact")
+    want = run_gpu(n, p, fused=0)
+    st = _capi.State(n)
+    st.set_options(fused=1, tune=1)
+    measured = 0
+    while True:
+        st.reset()
+        st.apply(p)
+        plan = st.last_plan()
+        got = st.amplitudes()
+        err = float(np.max(np.abs(got - want)))
+        say("measured plans 25q: tuning=%d sweeps=%d xcd=%d err=%.2e" % (plan["tuning"], plan["sweeps"],
+                                                                         plan["xcd_order_sweeps"], err))
+        assert err < TOL_FUSED
+        assert plan["xcd_order_sweeps"] <= plan["tile_sweeps"]
+        if not plan["tuning"]:
+            break
+        measured += 1
+        assert measured <= 4  # at most 3 candidate plans, then one run with the other tile order
+    assert measured >= 2
+    first = got
+    st.reset()
+    st.apply(p)
+    again = st.amplitudes()
+    plan2 = st.last_plan()
+    st.close()
+    assert plan2["tuning"] == 0 and plan2["xcd_order_sweeps"] == plan["xcd_order_sweeps"]
+    assert np.array_equal(first, again)
+
+
 def test_norm_and_expectation():
     rng = np.random.default_rng(77)
     n = 12
