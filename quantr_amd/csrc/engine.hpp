@@ -179,6 +179,7 @@ struct PlanParams {
   int max_ops = 0;    // ops per tile stage at most (0 = no limit)
   int lds_split = 0;  // remaps exchange the tile through LDS in two halves (half the LDS)
   int dyn_low = 0;    // moving layout: stages choose which bits the physical low bits hold
+  uint32_t seed = 0;  // 0: ties between equally good bits go to the lowest; else a seeded pick
 };
 
 // A remap done in two halves ("split", TileOpHost::flags bit 0) moves the amplitudes whose

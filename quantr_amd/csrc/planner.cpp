@@ -12,6 +12,7 @@
 // amplitudes through LDS so other tile bits become register bits. Diagonal gates and
 // controls may sit on any bit, in or out of the tile.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <exception>
@@ -194,17 +195,25 @@ uint64_t lookahead_phase_bits(int want, const std::vector<int>& cands, const std
   return best_set;
 }
 
+uint64_t mix64(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+
 // Greedy choice of `want` extra bits from `cands` on top of `base` maximising absorption.
+// Ties go to the first candidate, or with a nonzero `salt` to the one whose salted hash is
+// lowest (another plan of the same quality for the same program).
 uint64_t greedy_bits(uint64_t base, int want, const std::vector<int>& cands,
                      const std::vector<int>& remaining, const std::vector<Masks>& mk,
-                     const std::vector<uint8_t>& eligible) {
+                     const std::vector<uint8_t>& eligible, uint64_t salt = 0) {
   uint64_t cur = base;
   for (int step = 0; step < want; ++step) {
     int best = -1, best_bit = -1;
     for (int b : cands) {
       if (cur & (1ull << b)) continue;
       const int c = count_absorb(remaining, mk, eligible, cur | (1ull << b));
-      if (c > best) {
+      if (c > best || (salt && c == best && mix64(salt ^ (uint64_t)b) < mix64(salt ^ (uint64_t)best_bit))) {
         best = c;
         best_bit = b;
       }
@@ -423,7 +432,8 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
       for (uint32_t b = 0; b < n; ++b)
         if (phys[b] != (int)b || (int)b < c) B |= 1ull << b;
     int have = __builtin_popcountll(B);
-    if (have < k && !restore_only) B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible);
+    const uint64_t salt = pp.seed ? mix64(((uint64_t)pp.seed << 32) ^ stages.size()) | 1 : 0;
+    if (have < k && !restore_only) B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible, salt);
     // fill up if greedy ran out of useful bits (keeps k fixed for the kernel variant)
     for (uint32_t b = 0; b < n && __builtin_popcountll(B) < k; ++b) B |= 1ull << b;
     if (dyn && !restore_only && !identity_layout()) {
@@ -1005,25 +1015,39 @@ std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vec
     out.push_back(plan_fused(n, ops, pp, bytes_per_amp, dual));
     return out;
   }
+  // every cap with the first-candidate tie rule, and the caps that balance best with seeded
+  // tie rules: greedy choices that tie lead to plans of different sweep counts (on the
+  // 30-qubit brickwork, 23 to 28 sweeps over seeds), so a few seeds are planned as well
   static const int caps[] = {0, 160, 140, 120, 100, 90, 80};
-  constexpr int nc = (int)(sizeof(caps) / sizeof(caps[0]));
+  static const int seeded_caps[] = {0, 160, 140, 120, 100};
+  constexpr int kSeeds = 8;
+  std::vector<std::pair<int, uint32_t>> jobs;
+  for (int cap : caps) jobs.push_back({cap, 0u});
+  for (uint32_t seed = 1; seed < kSeeds; ++seed)
+    for (int cap : seeded_caps) jobs.push_back({cap, seed});
+  const int nc = (int)jobs.size();
   std::vector<std::vector<Stage>> plans(nc);
   std::vector<std::exception_ptr> errs(nc);
+  std::atomic<int> next{0};
+  const int nthreads = std::max(1, std::min<int>(nc, (int)std::thread::hardware_concurrency()));
   std::vector<std::thread> th;
-  for (int i = 0; i < nc; ++i)
-    th.emplace_back([&, i] {
-      try {
-        PlanParams q = pp;
-        q.max_ops = caps[i];
-        plans[i] = plan_fused(n, ops, q, bytes_per_amp, dual);
-      } catch (...) {
-        errs[i] = std::current_exception();
+  for (int t = 0; t < nthreads; ++t)
+    th.emplace_back([&] {
+      for (int i = next++; i < nc; i = next++) {
+        try {
+          PlanParams q = pp;
+          q.max_ops = jobs[i].first;
+          q.seed = jobs[i].second;
+          plans[i] = plan_fused(n, ops, q, bytes_per_amp, dual);
+        } catch (...) {
+          errs[i] = std::current_exception();
+        }
       }
     });
   for (auto& t : th) t.join();
   for (auto& e : errs)
     if (e) std::rethrow_exception(e);
-  // ascending estimate; the first of equal estimates (the lower cap index) first
+  // ascending estimate; the first of equal estimates (the earlier job) first
   std::vector<int> idx(nc);
   std::vector<double> cost(nc);
   for (int i = 0; i < nc; ++i) {

@@ -254,9 +254,9 @@ def test_brickwork_20q_vs_oracle():
 
 
 def test_balanced_plan_25q_vs_exact():
-    """From 2^25 amplitudes, forward plans are stage-balanced (several ops-per-stage caps are
-    planned and the lowest estimated duration kept); a caller-set cap is honoured as given.
-    Both match the reference-order mode."""
+    """From 2^25 amplitudes, forward plans are stage-balanced (several ops-per-stage caps and
+    tie rules are planned and the lowest estimated duration kept); a caller-set cap is
+    honoured as given. Both match the reference-order mode."""
     n = 25
     gates = W.brickwork(n, depth=40, seed=77)
     p = W.to_program(gates)
@@ -271,9 +271,8 @@ def test_balanced_plan_25q_vs_exact():
         plan = st.last_plan()
         st.close()
         assert np.max(np.abs(got - want)) < TOL_FUSED, jit
-        # the balancing chose a cap: no stage holds more gates than the largest cap tried (160)
         say("balanced 25q: largest stage %d gates, %d sweeps" % (plan["max_stage_gates"], plan["sweeps"]))
-        assert 0 < plan["max_stage_gates"] <= 160
+        assert 0 < plan["max_stage_gates"] and plan["sweeps"] <= len(gates) // 8
     say("balanced 25q stage_ops=7")
     st = _capi.State(n)
     st.set_options(fused=1, jit=0, stage_ops=7)
