@@ -354,7 +354,13 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
   cp->runs++;
   std::unique_ptr<StageTimer> timer;
   if (step >= 0) timer = std::make_unique<StageTimer>(stages.size());
-  const uint32_t measure_order = step >= 0 && step == (int)ts->cands.size() ? 1u : 0u;
+  // tiles dealt to the XCDs in contiguous eighths by default (faster on most sweeps of the
+  // 30-qubit brickwork); QR_TILE_ORDER=0 deals them in index order. Measuring applications
+  // try the default, then the other order.
+  uint32_t default_order = 1u;
+  if (const char* e = std::getenv("QR_TILE_ORDER")) default_order = e[0] == '0' ? 0u : 1u;
+  const uint32_t measure_order =
+      step >= 0 && step == (int)ts->cands.size() ? 1u - default_order : default_order;
   // ---- launch --------------------------------------------------------------------------
   qr_plan_stats ps{};
   for (size_t si = 0; si < stages.size(); ++si) {
@@ -367,7 +373,7 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
       const TileArgs a = tile_args(st, P, si);
       ProfScope ps_(st, "tile_sweep", s.bytes);
       if (use_jit) {
-        uint32_t order = step >= 0 ? measure_order : (orders ? (*orders)[si] : 0u);
+        uint32_t order = step >= 0 ? measure_order : (orders ? (*orders)[si] : default_order);
         if (persistent || (a.ntiles & 7) != 0) order = 0;
         ps.xcd_order_sweeps += order;
         hip_check(jit_launch(use->fns[si], s.k, pp.reg_bits, persistent, st->amp, a.coefs, a.ntiles,
@@ -399,9 +405,9 @@ void run_fast(qr_state* st, const std::vector<FastOp>& ops, bool fused, qr_plan_
         ts->chosen = (int)(std::min_element(ts->cand_ms.begin(), ts->cand_ms.end()) - ts->cand_ms.begin());
     } else {
       const std::vector<float>& base = ts->stage_ms[ts->chosen];
-      ts->order.assign(stages.size(), 0u);
+      ts->order.assign(stages.size(), default_order);
       for (size_t si = 0; si < stages.size(); ++si)
-        if (stages[si].tile && ms[si] < base[si]) ts->order[si] = 1u;
+        if (stages[si].tile && ms[si] < base[si]) ts->order[si] = measure_order;
     }
     ts->step++;
     ps.tuning = 1;

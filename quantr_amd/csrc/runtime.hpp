@@ -147,9 +147,10 @@ struct CachedPlan {
 };
 
 // Measured selection for one structure. Applications with qr_options.tune measure, in turn:
-// each candidate plan with tiles in index order (step 0..C-1), then the fastest candidate
-// with tiles dealt to the XCDs in contiguous eighths (step C). Each tile sweep then keeps the
-// faster of its two orders. `mu` serialises the measuring applications of the structure.
+// each candidate plan with the default tile order (step 0..C-1), then the fastest candidate
+// with the other order (step C; the orders are index order and the XCDs' contiguous eighths).
+// Each tile sweep then keeps the faster of its two orders. `mu` serialises the measuring
+// applications of the structure.
 struct TuneState {
   std::mutex mu;
   std::vector<std::shared_ptr<CachedPlan>> cands;  // [0] is the plan the estimate picks
@@ -157,7 +158,7 @@ struct TuneState {
   std::vector<std::vector<float>> stage_ms;        // per candidate, per stage (index order)
   int step = 0;                                    // next measurement; C + 1 = done
   int chosen = 0;
-  std::vector<uint32_t> order;  // per stage of the chosen plan: 0 index order, 1 per-XCD
+  std::vector<uint32_t> order;  // per stage of the chosen plan: 0 index order, 1 per-XCD eighths
 };
 
 std::shared_ptr<CachedPlan> get_plan(const qr_state* st, const std::vector<FastOp>& ops, const PlanParams& pp,
