@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--reg-bits", type=int, default=0)
     ap.add_argument("--stage-ops", type=int, default=0, help="gates per tile sweep at most (0 = auto)")
     ap.add_argument("--jit", type=int, default=2)
-    ap.add_argument("--no-tune", action="store_true",
-                    help="plans by estimate only (no measured selection of plans and tile orders)")
+    ap.add_argument("--tune", action="store_true",
+                    help="measured plans: time candidate plans and tile orders before the warmup")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--adjoint-qubits", type=int, default=20)
@@ -85,7 +85,7 @@ def bench_simulate(args):
     gates = W.brickwork(n, depth=args.depth, seed=1234)
     prog = W.to_program(gates).c_struct()
     st = _capi.State(n, prec)
-    tune = 0 if args.no_tune else 1
+    tune = 1 if args.tune else 0
     st.set_options(fused=1, tile_bits=args.tile_bits, low_bits=args.low_bits, tile_variant=args.tile_variant,
                    reg_bits=args.reg_bits, jit=args.jit, stage_ops=args.stage_ops, tune=tune)
     log("bench: %d-qubit %s brickwork depth %d, %d gates" % (n, args.precision, args.depth, len(gates)))
