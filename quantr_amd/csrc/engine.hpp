@@ -109,6 +109,11 @@ PauliZTerm z_term(const PauliMasks& tm);
 // The observable as device tables: terms grouped by X mask (i^ny folded into coefficients).
 void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
                   std::vector<PauliZTerm>& flat);
+// Orders an observable's G groups into expectation passes (no allocation): groups whose X
+// masks fit one tile are packed first-fit, widest first, into passes of one state read each;
+// groups too wide for a tile form a last, direct pass. Writes each group's X mask in its
+// pass's tile coordinates to lxm[] and returns the number of passes (at most G + 1).
+int pauli_passes(uint32_t n, int prec, PauliGroup* groups, int G, PauliPass* passes, uint32_t* lxm);
 
 // Validates and lowers a program. Throws Error.
 void validate_program(uint32_t n, const qr_program* p);

@@ -607,7 +607,8 @@ __global__ __launch_bounds__(256) void k_pauli_sum(typename CT<R>::T* __restrict
 template <typename R>
 __global__ __launch_bounds__(256) void k_pauli_expect(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
                                                       const PauliGroup* __restrict__ groups, int ngroups,
-                                                      const PauliZTerm* __restrict__ terms, double* __restrict__ part) {
+                                                      const PauliZTerm* __restrict__ terms, double* __restrict__ part,
+                                                      int accumulate) {
   __shared__ double red[256];
   double acc = 0.0;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
@@ -634,7 +635,125 @@ __global__ __launch_bounds__(256) void k_pauli_expect(const typename CT<R>::T* _
     if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) part[blockIdx.x] = accumulate ? part[blockIdx.x] + red[0] : red[0];
+}
+
+// The same partial sums for groups whose X masks all lie in one tile: tile bits = the set
+// bits of tmask (K of them, the lowest first). Each workgroup walks tiles blockIdx.x,
+// +gridDim.x, ...: it loads the tile into LDS once and takes every partner amplitude
+// psi[o ^ xm] from there (lxm[g] = xm of group g in tile coordinates), so the whole pass reads
+// the state once however many groups it holds.
+template <typename R, int K>
+__global__ __launch_bounds__(256) void k_pauli_expect_tiles(const typename CT<R>::T* __restrict__ psi,
+                                                            uint64_t ntiles, uint64_t tmask, uint64_t nontile,
+                                                            const PauliGroup* __restrict__ groups,
+                                                            const uint32_t* __restrict__ lxm, int ngroups,
+                                                            const PauliZTerm* __restrict__ terms,
+                                                            double* __restrict__ part, int accumulate) {
+  using T = typename CT<R>::T;
+  constexpr int NPT = (1 << K) / 256;  // amplitudes per thread
+  __shared__ T lds[1 << K];
+  __shared__ double red[256];
+  const uint32_t tid = threadIdx.x;
+  // global offset of tile position p = tid + 256 r: the thread's part (tile bits 0..7) and the
+  // register part (tile bits 8..K-1)
+  uint64_t toff = 0, hib[K - 8];
+  {
+    uint64_t m = tmask;
+    for (int b = 0; b < K; ++b) {
+      const uint64_t low = m & (0ull - m);
+      m ^= low;
+      if (b < 8) {
+        if ((tid >> b) & 1u) toff |= low;
+      } else {
+        hib[b - 8] = low;
+      }
+    }
+  }
+  double acc = 0.0;
+  // the next tile's amplitudes are loaded into registers while this one is evaluated from LDS
+  T nxt[NPT];
+  auto fetch = [&](uint64_t tile) {
+    const uint64_t b = deposit_bits(tile, nontile) | toff;
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) {
+      uint64_t off = 0;
+#pragma unroll
+      for (int q = 0; q < K - 8; ++q)
+        if ((r >> q) & 1) off |= hib[q];
+      const T* src = &psi[b | off];
+      nxt[r].x = src->x;
+      nxt[r].y = src->y;
+    }
+  };
+  if ((uint64_t)blockIdx.x < ntiles) fetch(blockIdx.x);
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = deposit_bits(t, nontile) | toff;
+#pragma unroll
+    for (int r = 0; r < NPT; ++r) lds[tid + 256 * r] = nxt[r];
+    __syncthreads();
+    if (t + gridDim.x < ntiles) fetch(t + gridDim.x);
+    // per group: the coefficient of every position (terms outer, the thread's positions inner),
+    // then the partner amplitudes from LDS. The sign (-1)^{popcount(j & zm)}, j = o ^ xm, splits
+    // into the thread's part (base ^ xm) and the register part of o: r's bits on the tile bits
+    // 8.. that zm holds (hz), a constant per position.
+    constexpr int CH = 16;  // positions per chunk (register budget)
+#pragma unroll 1
+    for (int c0 = 0; c0 < NPT; c0 += CH) {
+      double yr[CH], yi[CH];
+#pragma unroll
+      for (int r = 0; r < CH; ++r) yr[r] = yi[r] = 0.0;
+      for (int g = 0; g < ngroups; ++g) {
+        const uint64_t bx = base ^ groups[g].xm;
+        const uint32_t lx = lxm[g];
+        const uint32_t t0 = groups[g].first, t1 = t0 + groups[g].count;
+        double fr[CH], fi[CH];
+#pragma unroll
+        for (int r = 0; r < CH; ++r) fr[r] = fi[r] = 0.0;
+        for (uint32_t q = t0; q < t1; ++q) {
+          const uint64_t zm = terms[q].zm;
+          const double cr = terms[q].cr, ci = terms[q].ci;
+          uint32_t hz = 0;
+#pragma unroll
+          for (int b = 0; b < K - 8; ++b)
+            if (zm & hib[b]) hz |= 1u << b;
+          const uint32_t bt = (uint32_t)__popcll(bx & zm) & 1u;
+#pragma unroll
+          for (int r = 0; r < CH; ++r) {
+            const uint32_t neg = bt ^ ((uint32_t)__popc((uint32_t)(c0 + r) & hz) & 1u);
+            fr[r] += neg ? -cr : cr;
+          }
+          if (ci != 0.0) {
+#pragma unroll
+            for (int r = 0; r < CH; ++r) {
+              const uint32_t neg = bt ^ ((uint32_t)__popc((uint32_t)(c0 + r) & hz) & 1u);
+              fi[r] += neg ? -ci : ci;
+            }
+          }
+        }
+#pragma unroll
+        for (int r = 0; r < CH; ++r) {
+          const T w = lds[(tid + 256u * (uint32_t)(c0 + r)) ^ lx];
+          const double pr = (double)w.x, pi = (double)w.y;
+          yr[r] += fr[r] * pr - fi[r] * pi;
+          yi[r] += fr[r] * pi + fi[r] * pr;
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < CH; ++r) {
+        const T v = lds[tid + 256u * (uint32_t)(c0 + r)];
+        acc += (double)v.x * yr[r] + (double)v.y * yi[r];
+      }
+    }
+    __syncthreads();  // the next tile overwrites lds
+  }
+  red[tid] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)tid < w) red[tid] += red[tid + w];
+    __syncthreads();
+  }
+  if (tid == 0) part[blockIdx.x] = accumulate ? part[blockIdx.x] + red[0] : red[0];
 }
 
 // Partial sums of Re sum_j conj(a_j) b_j, one per workgroup.
@@ -896,12 +1015,45 @@ hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, 
 }
 
 hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
-                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s) {
+                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s, int accumulate) {
   if (prec == 0)
-    dev::k_pauli_expect<double><<<nblocks, 256, 0, s>>>((const double2*)psi, dim, groups, ngroups, terms, part);
+    dev::k_pauli_expect<double>
+        <<<nblocks, 256, 0, s>>>((const double2*)psi, dim, groups, ngroups, terms, part, accumulate);
   else
-    dev::k_pauli_expect<float><<<nblocks, 256, 0, s>>>((const float2*)psi, dim, groups, ngroups, terms, part);
+    dev::k_pauli_expect<float>
+        <<<nblocks, 256, 0, s>>>((const float2*)psi, dim, groups, ngroups, terms, part, accumulate);
   return hipGetLastError();
+}
+
+int pauli_expect_partials(uint64_t dim) { return (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, dim / 256)); }
+
+hipError_t launch_pauli_expect_passes(int prec, const void* psi, uint64_t dim, const PauliGroup* groups,
+                                      const uint32_t* lxm, const PauliZTerm* terms, const PauliPass* passes,
+                                      int npasses, double* part, hipStream_t s) {
+  const int nb = pauli_expect_partials(dim);
+  for (int i = 0; i < npasses; ++i) {
+    const PauliPass& ps = passes[i];
+    const int acc = i > 0 ? 1 : 0;
+    if (ps.tmask == 0) {
+      const hipError_t e = launch_pauli_expect(prec, psi, dim, groups + ps.first, (int)ps.count, terms, part, nb, s, acc);
+      if (e != hipSuccess) return e;
+      continue;
+    }
+    const int K = __builtin_popcountll(ps.tmask);
+    if (K != pauli_tile_bits(prec) || (dim >> K) == 0) return hipErrorInvalidValue;
+    const uint64_t ntiles = dim >> K, nontile = (dim - 1) & ~ps.tmask;
+    if (prec == 0)
+      dev::k_pauli_expect_tiles<double, 12><<<nb, 256, 0, s>>>((const double2*)psi, ntiles, ps.tmask, nontile,
+                                                              groups + ps.first, lxm + ps.first, (int)ps.count,
+                                                              terms, part, acc);
+    else
+      dev::k_pauli_expect_tiles<float, 13><<<nb, 256, 0, s>>>((const float2*)psi, ntiles, ps.tmask, nontile,
+                                                             groups + ps.first, lxm + ps.first, (int)ps.count,
+                                                             terms, part, acc);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,

@@ -127,9 +127,29 @@ struct PauliGroup {
 // (the whole observable in one pass).
 hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
                             const PauliZTerm* terms, hipStream_t s);
-// part[b] = partial sums of Re <psi| O |psi> over nblocks blocks, O as in launch_pauli_sum.
+// part[b] = partial sums of Re <psi| O |psi> over nblocks blocks, O as in launch_pauli_sum
+// (added to part[b] with `accumulate`). Every group costs a read of the state.
 hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
-                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s);
+                               const PauliZTerm* terms, double* part, int nblocks, hipStream_t s,
+                               int accumulate = 0);
+// Groups of an observable evaluated together in one read of the state: groups[first ..
+// first + count) whose X masks lie in the tile bits tmask (pauli_tile_bits of them), or with
+// tmask == 0 groups read directly (X masks too wide for a tile).
+struct PauliPass {
+  uint64_t tmask;
+  uint32_t first, count;
+};
+// Tile width of the grouped expectation passes: 2^12 complex128 / 2^13 complex64 amplitudes
+// (64 KB of LDS), of which the lowest 3 / 4 bits (128-byte segments) are always tile bits.
+inline int pauli_tile_bits(int prec) { return prec == 0 ? 12 : 13; }
+inline int pauli_low_bits(int prec) { return prec == 0 ? 3 : 4; }
+// Partial sums (workgroups) of every expectation pass over a state of dim amplitudes.
+int pauli_expect_partials(uint64_t dim);
+// part[0 .. pauli_expect_partials(dim)) = partial sums of Re <psi|O|psi>, pass after pass;
+// lxm[g] = group g's X mask in the tile coordinates of its pass.
+hipError_t launch_pauli_expect_passes(int prec, const void* psi, uint64_t dim, const PauliGroup* groups,
+                                      const uint32_t* lxm, const PauliZTerm* terms, const PauliPass* passes,
+                                      int npasses, double* part, hipStream_t s);
 // part[b] = partial sums of Re sum_j conj(a_j) b_j over nblocks blocks.
 hipError_t launch_dot_re_partial(int prec, const void* a, const void* b, uint64_t dim, double* part, int nblocks,
                                  hipStream_t s);

@@ -475,6 +475,60 @@ PauliZTerm z_term(const PauliMasks& tm) {
   return PauliZTerm{tm.zm, cr, ci};
 }
 
+int pauli_passes(uint32_t n, int prec, PauliGroup* groups, int G, PauliPass* passes, uint32_t* lxm) {
+  const int K = pauli_tile_bits(prec), c = pauli_low_bits(prec);
+  const uint64_t low = (1ull << c) - 1;
+  if (G <= 0) return 0;
+  if ((int)n < K + 3) {  // small states: one direct pass
+    passes[0] = PauliPass{0, 0, (uint32_t)G};
+    for (int i = 0; i < G; ++i) lxm[i] = 0;
+    return 1;
+  }
+  auto hi = [&](const PauliGroup& x) { return __builtin_popcountll(x.xm & ~low); };
+  // groups whose X mask fits a tile first, widest first (then by mask): first-fit packs them
+  std::sort(groups, groups + G, [&](const PauliGroup& a, const PauliGroup& b) {
+    const bool wa = hi(a) > K - c, wb = hi(b) > K - c;
+    if (wa != wb) return wb;
+    if (hi(a) != hi(b)) return hi(a) > hi(b);
+    return a.xm < b.xm;
+  });
+  int narrow = 0;
+  while (narrow < G && hi(groups[narrow]) <= K - c) ++narrow;
+  int np = 0, done = 0;
+  while (done < narrow) {
+    uint64_t uni = groups[done].xm & ~low;
+    int end = done + 1;
+    for (int i = done + 1; i < narrow; ++i) {
+      const uint64_t u2 = uni | (groups[i].xm & ~low);
+      if (__builtin_popcountll(u2) <= K - c) {
+        std::swap(groups[end], groups[i]);
+        ++end;
+        uni = u2;
+      }
+    }
+    // tile bits: the low segment bits, the groups' X bits, then the lowest other bits
+    uint64_t t = low | uni;
+    for (uint32_t b = 0; b < n && __builtin_popcountll(t) < K; ++b) t |= 1ull << b;
+    for (int i = done; i < end; ++i) {
+      uint32_t lx = 0;
+      uint64_t m = t;
+      for (int b = 0; b < K; ++b) {
+        const uint64_t bit = m & (0ull - m);
+        m ^= bit;
+        if (groups[i].xm & bit) lx |= 1u << b;
+      }
+      lxm[i] = lx;
+    }
+    passes[np++] = PauliPass{t, (uint32_t)done, (uint32_t)(end - done)};
+    done = end;
+  }
+  if (narrow < G) {
+    for (int i = narrow; i < G; ++i) lxm[i] = 0;
+    passes[np++] = PauliPass{0, (uint32_t)narrow, (uint32_t)(G - narrow)};
+  }
+  return np;
+}
+
 void pauli_tables(const std::vector<PauliMasks>& terms, std::vector<PauliGroup>& groups,
                   std::vector<PauliZTerm>& flat) {
   // terms grouped by X mask; Z masks and i^ny folded into per-term complex coefficients

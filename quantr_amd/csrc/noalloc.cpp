@@ -1,12 +1,12 @@
 // noalloc.cpp — qr_expectation_noalloc: <psi|O|psi> with no device or host allocation.
 //
 // Callers that cannot allocate (code under HIP graph capture, tight serving loops, harnesses
-// that forbid hipMalloc) get the expectation value through the same one-pass kernel as
-// qr_expectation, with its device working set (the observable's X-mask groups, its Z-mask
-// terms and the per-workgroup partial sums) in a fixed device array instead of workspaces that
-// grow by hipMalloc. Host staging lives in fixed static arrays as well. Results are bitwise
-// equal to qr_expectation's: same tables in the same order, same kernel, same partial count and
-// the same host summation.
+// that forbid hipMalloc) get the expectation value through the same passes and kernels as
+// qr_expectation, with its device working set (the observable's X-mask groups, their tile
+// masks, its Z-mask terms and the per-workgroup partial sums) in a fixed device array instead
+// of workspaces that grow by hipMalloc. Host staging lives in fixed static arrays as well.
+// Results are bitwise equal to qr_expectation's: same tables in the same order, same passes,
+// same kernels, same partial count and the same host summation.
 //
 // The device array is one per GPU (each GPU holds its own copy until the process ends). Every
 // call on a GPU shares it and it keeps the last call's contents; nothing in it needs to start
@@ -32,8 +32,10 @@ constexpr uint64_t kMaxTerms = 16384;         // observable terms at most
 constexpr int kMaxParts = 1024;               // partial sums (one per workgroup)
 constexpr int kMaxDevices = 64;
 
-// groups (16 B) and terms (24 B) of at most kMaxTerms terms, then the partials
-static_assert(kMaxTerms * (sizeof(PauliGroup) + sizeof(PauliZTerm)) + kMaxParts * sizeof(double) + 256 <=
+// groups (16 B), their tile masks (4 B) and terms (24 B) of at most kMaxTerms terms, then the
+// partials
+static_assert(kMaxTerms * (sizeof(PauliGroup) + sizeof(uint32_t) + sizeof(PauliZTerm)) + 8 +
+                      kMaxParts * sizeof(double) + 256 <=
                   kScratchBytes,
               "scratch layout");
 
@@ -56,6 +58,8 @@ struct KeyIdx {
 };
 KeyIdx g_keys[kMaxTerms];
 PauliZTerm g_terms_in[kMaxTerms];
+PauliPass g_passes[kMaxTerms + 1];
+uint32_t g_lxm[kMaxTerms];
 double g_parts[kMaxParts];
 
 // The device address of this GPU's copy of `scratch`, looked up once per device with that
@@ -105,15 +109,20 @@ double expectation_noalloc(qr_state* st, const qr_observable* o) {
     const uint64_t end = g + 1 < G ? groups[g + 1].first : T;
     groups[g].count = (uint32_t)(end - groups[g].first);
   }
-  const size_t gb = (size_t)G * sizeof(PauliGroup);
-  PauliZTerm* flat = (PauliZTerm*)(g_stage + gb);
+  // the same passes as qr_expectation (groups reordered in place)
+  const int np = pauli_passes(st->n, st->prec, groups, (int)G, g_passes, g_lxm);
+  const size_t gb = (size_t)G * sizeof(PauliGroup), lb = ((size_t)G * 4 + 7) & ~(size_t)7;
+  std::memcpy(g_stage + gb, g_lxm, (size_t)G * 4);
+  PauliZTerm* flat = (PauliZTerm*)(g_stage + gb + lb);
   for (uint64_t t = 0; t < T; ++t) flat[t] = g_terms_in[g_keys[t].idx];
   const size_t tb = (size_t)T * sizeof(PauliZTerm);
-  const size_t part_off = (gb + tb + 255) & ~(size_t)255;
-  const int nb = (int)std::min<uint64_t>(kMaxParts, std::max<uint64_t>(1, st->dim / 256));
-  hip_check(hipMemcpyAsync(dev, g_stage, gb + tb, hipMemcpyHostToDevice, st->stream), "hipMemcpyAsync H2D");
-  hip_check(launch_pauli_expect(st->prec, st->amp, st->dim, (const PauliGroup*)dev, (int)G,
-                                (const PauliZTerm*)(dev + gb), (double*)(dev + part_off), nb, st->stream),
+  const size_t part_off = (gb + lb + tb + 255) & ~(size_t)255;
+  const int nb = pauli_expect_partials(st->dim);
+  static_assert(kMaxParts >= 1024, "pauli_expect_partials returns at most 1024");
+  hip_check(hipMemcpyAsync(dev, g_stage, gb + lb + tb, hipMemcpyHostToDevice, st->stream), "hipMemcpyAsync H2D");
+  hip_check(launch_pauli_expect_passes(st->prec, st->amp, st->dim, (const PauliGroup*)dev,
+                                       (const uint32_t*)(dev + gb), (const PauliZTerm*)(dev + gb + lb), g_passes,
+                                       np, (double*)(dev + part_off), st->stream),
             "expectation");
   hip_check(hipMemcpyAsync(g_parts, dev + part_off, nb * sizeof(double), hipMemcpyDeviceToHost, st->stream), "D2H");
   hip_check(hipStreamSynchronize(st->stream), "sync");

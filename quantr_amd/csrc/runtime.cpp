@@ -573,15 +573,23 @@ double expectation(qr_state* st, const std::vector<PauliMasks>& terms) {
   std::vector<PauliGroup> groups;
   std::vector<PauliZTerm> flat;
   pauli_tables(terms, groups, flat);
-  const size_t gb = groups.size() * sizeof(PauliGroup), tb = flat.size() * sizeof(PauliZTerm);
-  std::vector<uint8_t> blob(gb + tb);
+  const int G = (int)groups.size();
+  std::vector<PauliPass> passes(G + 1);
+  std::vector<uint32_t> lxm(G);
+  const int np = pauli_passes(st->n, st->prec, groups.data(), G, passes.data(), lxm.data());
+  // device tables: groups, their tile masks (padded to 8 bytes), terms
+  const size_t gb = (size_t)G * sizeof(PauliGroup), lb = ((size_t)G * 4 + 7) & ~(size_t)7;
+  const size_t tb = flat.size() * sizeof(PauliZTerm);
+  std::vector<uint8_t> blob(gb + lb + tb, 0);
   std::memcpy(blob.data(), groups.data(), gb);
-  if (tb) std::memcpy(blob.data() + gb, flat.data(), tb);
+  std::memcpy(blob.data() + gb, lxm.data(), (size_t)G * 4);
+  if (tb) std::memcpy(blob.data() + gb + lb, flat.data(), tb);
   uint8_t* d = upload(st, st->aux, blob);
-  const int nb = (int)std::min<uint64_t>(1024, std::max<uint64_t>(1, st->dim / 256));
+  const int nb = pauli_expect_partials(st->dim);
   st->aux2.reserve(nb * sizeof(double));
-  hip_check(launch_pauli_expect(st->prec, st->amp, st->dim, (const PauliGroup*)d, (int)groups.size(),
-                                (const PauliZTerm*)(d + gb), (double*)st->aux2.p, nb, st->stream),
+  hip_check(launch_pauli_expect_passes(st->prec, st->amp, st->dim, (const PauliGroup*)d, (const uint32_t*)(d + gb),
+                                       (const PauliZTerm*)(d + gb + lb), passes.data(), np, (double*)st->aux2.p,
+                                       st->stream),
             "expectation");
   std::vector<double> h(nb);
   hip_check(hipMemcpyAsync(h.data(), st->aux2.p, nb * sizeof(double), hipMemcpyDeviceToHost, st->stream), "D2H");

@@ -5,6 +5,7 @@
 // must be rejected with an Error, never read out of bounds.
 #include <cstdio>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 #include <random>
 #include <vector>
@@ -106,6 +107,66 @@ void run_all_planners(uint32_t n, const Prog& p) {
   pd.reg_bits = std::min(4, pd.tile_bits);
   pd.low_bits = 3;
   (void)plan_fused(n, rev, pd, 16, true);
+}
+
+// pauli_passes: every group lands in exactly one pass; a tiled pass spans K bits including
+// the low segment bits and its groups' X masks, with lxm the masks in tile coordinates; only
+// groups too wide for a tile are read directly.
+void check_pauli_passes(std::mt19937_64& rng) {
+  for (int trial = 0; trial < 200; ++trial) {
+    const uint32_t n = 1 + (uint32_t)(rng() % 34);
+    const int prec = (int)(rng() % 2);
+    const int G = 1 + (int)(rng() % 120);
+    const uint64_t dimmask = n >= 64 ? ~0ull : (1ull << n) - 1;
+    std::vector<PauliGroup> g(G);
+    for (int i = 0; i < G; ++i) {
+      uint64_t xm = 0;
+      const int kind = (int)(rng() % 4);
+      if (kind == 1) xm = 1ull << (rng() % n);
+      if (kind == 2) xm = (1ull << (rng() % n)) | (1ull << (rng() % n));
+      if (kind == 3) xm = rng() & dimmask;
+      g[i] = PauliGroup{xm, (uint32_t)i, 1u};
+    }
+    std::vector<PauliGroup> orig = g;
+    std::vector<PauliPass> passes(G + 1);
+    std::vector<uint32_t> lxm(G);
+    const int np = pauli_passes(n, prec, g.data(), G, passes.data(), lxm.data());
+    const int K = pauli_tile_bits(prec), c = pauli_low_bits(prec);
+    const uint64_t low = (1ull << c) - 1;
+    uint32_t next = 0;
+    bool ok = np >= 1 && np <= G + 1;
+    for (int p = 0; ok && p < np; ++p) {
+      ok = passes[p].first == next && passes[p].count > 0;
+      next += passes[p].count;
+      const uint64_t t = passes[p].tmask;
+      if (t != 0) {
+        ok = ok && __builtin_popcountll(t) == K && (t & low) == low && (t & ~dimmask) == 0;
+        for (uint32_t i = passes[p].first; ok && i < passes[p].first + passes[p].count; ++i) {
+          ok = (g[i].xm & ~t) == 0;
+          uint32_t lx = 0;
+          uint64_t m = t;
+          for (int b = 0; b < K; ++b) {
+            const uint64_t bit = m & (0ull - m);
+            m ^= bit;
+            if (g[i].xm & bit) lx |= 1u << b;
+          }
+          ok = ok && lx == lxm[i];
+        }
+      } else if ((int)n >= K + 3) {
+        for (uint32_t i = passes[p].first; ok && i < passes[p].first + passes[p].count; ++i)
+          ok = __builtin_popcountll(g[i].xm & ~low) > K - c;
+      }
+    }
+    ok = ok && next == (uint32_t)G;
+    auto key = [](const PauliGroup& a, const PauliGroup& b) { return a.first < b.first; };
+    std::sort(g.begin(), g.end(), key);
+    for (int i = 0; ok && i < G; ++i) ok = g[i].xm == orig[i].xm;
+    if (!ok) {
+      std::printf("FAIL: pauli_passes n=%u prec=%d G=%d\n", n, prec, G);
+      ++failures;
+      return;
+    }
+  }
 }
 
 }  // namespace
@@ -244,6 +305,7 @@ int main() {
   reset();
   o.wires = nullptr;
   expect_error("wires NULL", [](void*) { (void)parse_observable(3, &o); }, nullptr);
+  check_pauli_passes(rng);
   if (failures) return 1;
   std::printf("sanitize driver: ok\n");
   return 0;

@@ -4,6 +4,7 @@ Bars: the unfused reference-order path (fused=0) is bitwise equal to the oracle;
 path is within 1e-12 max amplitude error (complex128); complex64 within 2e-5; bins identical
 for the same injected draws.
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -335,6 +336,45 @@ def test_norm_and_expectation():
     want = H.oracle_expectation(n, psi, obs)
     assert abs(got - want) < 1e-11
     st.close()
+
+
+def _pass_observable(n, rng):
+    """Terms that exercise every kind of expectation pass: Z strings (X mask 0), single and
+    paired X/Y factors on low and high wires (grouped into tile passes), X masks spread over
+    more wires than a tile holds (the direct pass), and repeated X masks with different Z
+    parts (one group, several terms)."""
+    terms = list(W.hea_observable(n))
+    for _ in range(12):
+        w = sorted(rng.choice(n, size=2, replace=False).tolist())
+        terms.append((float(rng.normal()), [(w[0], str(rng.choice(["X", "Y"]))), (w[1], str(rng.choice(["X", "Y", "Z"])))]))
+    wide = sorted(rng.choice(n, size=min(n, 13), replace=False).tolist())
+    terms.append((0.37, [(w, "X") for w in wide]))
+    terms.append((-0.21, [(w, "Y" if i % 3 == 0 else "X") for i, w in enumerate(wide[:11])]))
+    for w in (0, n - 1, n // 2):
+        terms.append((0.11, [(w, "X"), ((w + 1) % n, "Z")]))
+        terms.append((-0.07, [(w, "X"), ((w + 2) % n, "Z"), ((w + 3) % n, "Z")]))
+    return _capi.Observable(terms)
+
+
+@pytest.mark.parametrize("precision,n,tol", [(_capi.QR_C128, 17, 1e-11), (_capi.QR_C64, 18, 2e-5)])
+def test_expectation_passes(precision, n, tol):
+    """Expectation values of states large enough for tiled passes (X-mask groups evaluated
+    together from one read of the state, wide groups read directly) against the oracle, and
+    the allocation-free entry point bitwise equal to qr_expectation."""
+    rng = np.random.default_rng(700 + n)
+    st = _capi.State(n, precision)
+    st.apply(W.to_program(W.brickwork(n, depth=8, seed=n)))
+    psi = st.amplitudes()  # the amplitudes the state holds (rounded for complex64)
+    obs = _pass_observable(n, rng)
+    got = st.expectation(obs)
+    want = H.oracle_expectation(n, psi, obs)
+    say("expectation passes n=%d prec=%d: got %.15f want %.15f" % (n, precision, got, want))
+    scale = sum(abs(c) for c, _ in obs.terms)
+    assert abs(got - want) < tol * max(1.0, scale)
+    v = C.c_double()
+    _capi.check(_capi.lib().qr_expectation_noalloc(st.handle, C.byref(obs.c_struct()), C.byref(v)))
+    st.close()
+    assert v.value == got
 
 
 def _energy_oracle(n, gates, obs):
