@@ -266,11 +266,12 @@ qr_status qr_grad_set_options(qr_grad* grad, const qr_options* options);
 typedef struct qr_batch qr_batch;
 /* Batch options (NULL = defaults: all zero).
  *  autotune        1: for every qubit count >= 16 in the batch, time the candidate tile widths
- *                  ("fusion widths") of the reverse sweep on one representative instance and
- *                  keep the fastest (cached per process); 0: built-in widths
+ *                  ("fusion widths") of the forward sweep, then of the reverse sweep, on one
+ *                  representative instance and keep the fastest (cached per process; widths
+ *                  set below are kept); 0: built-in widths
  *  fwd_tile_bits   forward-sweep tile width (0 = auto)
  *  dual_tile_bits  reverse-sweep tile width (0 = auto or tuned)
- *  streams         concurrent streams inside the graph (0 = 8) */
+ *  streams         concurrent streams inside the graph (0 = 4) */
 typedef struct qr_batch_options {
   uint32_t autotune;
   uint32_t fwd_tile_bits;

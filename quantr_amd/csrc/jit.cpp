@@ -496,6 +496,133 @@ void compile_in_children(const std::vector<std::string>& todo, std::vector<Group
   }
 }
 
+// ---- code objects on disk ---------------------------------------------------------------
+// Compiled stages outlive the process: each group's code object is stored once in a cache
+// directory, with one small index file per kernel (named by a hash of the kernel source and of
+// what the compiler made of it: the tile-ops text, the options, the compiler binary's size and
+// time) giving the object file and the kernel's name in it. Writes go to a temporary name
+// and are renamed into place, so concurrent processes never read a partial file. Any cache
+// error just means compiling. QR_JIT_CACHE=0 disables the cache, QR_JIT_CACHE=<dir> moves it;
+// the default is $XDG_CACHE_HOME/quantr_amd or ~/.cache/quantr_amd.
+
+struct Hash128 {
+  uint64_t a = 0xcbf29ce484222325ull, b = 0x84222325cbf29ce4ull;
+  void add(const std::string& s) {
+    for (unsigned char ch : s) {
+      a = (a ^ ch) * 0x100000001b3ull;
+      b = (b ^ ch) * 0x9e3779b97f4a7c15ull;
+      b ^= b >> 29;
+    }
+    a ^= s.size();
+    b += s.size() * 0xbf58476d1ce4e5b9ull;
+  }
+  std::string hex() const {
+    char buf[40];
+    std::snprintf(buf, sizeof buf, "%016llx%016llx", (unsigned long long)a, (unsigned long long)b);
+    return buf;
+  }
+};
+
+const char* const kCompileOptions = "-x hip --offload-arch=gfx950 --cuda-device-only -O3 -std=c++17 "
+                                    "-target-feature -packed-fp32-ops";
+
+std::string cache_dir() {
+  static std::string dir = [] {
+    const char* e = std::getenv("QR_JIT_CACHE");
+    std::string base;
+    if (e && *e) {
+      if (std::string(e) == "0") return std::string();
+      base = e;
+    } else {
+      const char* x = std::getenv("XDG_CACHE_HOME");
+      const char* h = std::getenv("HOME");
+      if (x && *x) base = std::string(x) + "/quantr_amd";
+      else if (h && *h) base = std::string(h) + "/.cache/quantr_amd";
+      else return std::string();
+    }
+    // one directory per toolchain identity
+    Hash128 id;
+    id.add(kTileOpsSource);
+    id.add(kCompileOptions);
+    const std::string cc = compiler_path();
+    struct stat sb{};
+    if (!cc.empty() && stat(cc.c_str(), &sb) == 0)
+      id.add(cc + ":" + std::to_string((long long)sb.st_size) + ":" + std::to_string((long long)sb.st_mtime));
+    std::string d = base + "/jit-" + id.hex().substr(0, 16);
+    // create the path one level at a time
+    for (size_t at = 1; at != std::string::npos;) {
+      at = d.find('/', at + 1);
+      const std::string part = d.substr(0, at);
+      if (mkdir(part.c_str(), 0755) != 0 && errno != EEXIST) return std::string();
+    }
+    return access(d.c_str(), W_OK) == 0 ? d : std::string();
+  }();
+  return dir;
+}
+
+std::string kernel_key(const std::string& kernel) {
+  Hash128 h;
+  h.add(kernel);
+  return h.hex();
+}
+
+bool write_file_atomic(const std::string& path, const char* data, size_t size) {
+  const std::string tmp = path + ".tmp" + std::to_string((long long)getpid());
+  {
+    std::ofstream f(tmp, std::ios::binary);
+    if (!f) return false;
+    f.write(data, (std::streamsize)size);
+    if (!f) {
+      (void)unlink(tmp.c_str());
+      return false;
+    }
+  }
+  if (rename(tmp.c_str(), path.c_str()) != 0) {
+    (void)unlink(tmp.c_str());
+    return false;
+  }
+  return true;
+}
+
+std::map<std::string, hipModule_t> g_disk_modules;  // object file -> module (under g_mu)
+
+// The kernel for `kernel` from the disk cache, if there. Caller holds no lock.
+bool load_cached(const std::string& dir, const std::string& kernel, JitKernel& out) {
+  std::ifstream f(dir + "/" + kernel_key(kernel) + ".k");
+  std::string obj, name;
+  if (!f || !std::getline(f, obj) || !std::getline(f, name) || obj.empty() || name.empty()) return false;
+  if (obj.find('/') != std::string::npos) return false;
+  std::lock_guard<std::mutex> lk(g_mu);
+  hipModule_t mod = nullptr;
+  auto it = g_disk_modules.find(obj);
+  if (it != g_disk_modules.end()) {
+    mod = it->second;
+  } else {
+    std::vector<char> code;
+    if (!read_file(dir + "/" + obj, code) || hipModuleLoadData(&mod, code.data()) != hipSuccess) return false;
+    g_disk_modules[obj] = mod;
+  }
+  hipFunction_t fn = nullptr;
+  if (hipModuleGetFunction(&fn, mod, name.c_str()) != hipSuccess) return false;
+  out = JitKernel{mod, fn};
+  return true;
+}
+
+// Stores a compiled group and indexes each of its kernels.
+void store_group(const std::string& dir, const std::vector<std::string>& todo, const Group& G,
+                 const std::vector<char>& code) {
+  Hash128 gh;
+  for (size_t j = 0; j < G.items.size(); ++j) gh.add(todo[G.items[j]]);
+  const std::string obj = "g" + gh.hex() + ".co";
+  if (access((dir + "/" + obj).c_str(), R_OK) != 0 && !write_file_atomic(dir + "/" + obj, code.data(), code.size()))
+    return;
+  for (size_t j = 0; j < G.items.size(); ++j) {
+    const std::string& k = todo[G.items[j]];
+    const std::string idx = obj + "\n" + entry_name(k) + "_" + std::to_string(j) + "\n";
+    (void)write_file_atomic(dir + "/" + kernel_key(k) + ".k", idx.data(), idx.size());
+  }
+}
+
 }  // namespace
 
 void jit_prepare(const std::vector<std::string>& sources) {
@@ -505,6 +632,27 @@ void jit_prepare(const std::vector<std::string>& sources) {
     for (const auto& s : sources)
       if (!s.empty() && !g_cache.count(s) && std::find(todo.begin(), todo.end(), s) == todo.end()) todo.push_back(s);
   }
+  if (todo.empty()) return;
+  // kernels already on disk
+  const std::string dir = cache_dir();
+  size_t from_disk = 0;
+  if (!dir.empty()) {
+    std::vector<std::string> left;
+    for (const auto& s : todo) {
+      JitKernel k;
+      if (load_cached(dir, s, k)) {
+        std::lock_guard<std::mutex> lk(g_mu);
+        if (!g_cache.count(s)) g_cache[s] = k;
+        ++from_disk;
+      } else {
+        left.push_back(s);
+      }
+    }
+    todo.swap(left);
+  }
+  if (const char* e = std::getenv("QR_JIT_CACHE_LOG"))
+    if (e[0] == '1')
+      std::fprintf(stderr, "[qr jit] %zu kernels from the disk cache, %zu to compile\n", from_disk, todo.size());
   if (todo.empty()) return;
   std::vector<char> done(todo.size(), 0);
   std::vector<JitKernel> made(todo.size());
@@ -518,6 +666,7 @@ void jit_prepare(const std::vector<std::string>& sources) {
     for (auto& G : groups) {
       std::vector<char> code;
       if (G.ok && read_file(G.obj, code)) {
+        if (!dir.empty()) store_group(dir, todo, G, code);
         hipModule_t mod = nullptr;
         if (hipModuleLoadData(&mod, code.data()) == hipSuccess) {
           for (size_t j = 0; j < G.items.size(); ++j) {
@@ -544,6 +693,13 @@ void jit_prepare(const std::vector<std::string>& sources) {
     std::string err;
     if (!compile_one(full_source(todo[i]), code, err))
       throw Error{QR_ERR_HIP, "runtime compilation of a tile stage failed: " + err};
+    if (!dir.empty()) {
+      const std::string key = kernel_key(todo[i]), obj = "s" + key + ".co";
+      if (write_file_atomic(dir + "/" + obj, code.data(), code.size())) {
+        const std::string idx = obj + "\n" + entry_name(todo[i]) + "\n";
+        (void)write_file_atomic(dir + "/" + key + ".k", idx.data(), idx.size());
+      }
+    }
     JitKernel k;
     hipError_t e = hipModuleLoadData(&k.mod, code.data());
     if (e != hipSuccess) throw Error{QR_ERR_HIP, std::string("hipModuleLoadData: ") + hipGetErrorString(e)};

@@ -322,6 +322,40 @@ def test_measured_plans_25q():
     assert np.array_equal(first, again)
 
 
+_CACHE_SCRIPT = r"""
+import hashlib, sys
+sys.path.insert(0, sys.argv[1])
+from quantr_amd import _capi, workloads as W
+st = _capi.State(20)
+st.set_options(fused=1, jit=1)
+st.apply(W.to_program(W.brickwork(20, depth=12, seed=2020)))
+print("digest", hashlib.sha256(st.amplitudes().tobytes()).hexdigest(), flush=True)
+"""
+
+
+def test_stage_cache_on_disk(tmp_path):
+    """Compiled stages are kept on disk: a second process with the same cache directory loads
+    every stage of the same circuit from it, compiles none, and gets the same amplitudes bit for
+    bit."""
+    import os
+    import subprocess
+    env = dict(os.environ, QR_JIT_CACHE=str(tmp_path / "jit"), QR_JIT_CACHE_LOG="1")
+    runs = []
+    for k in range(2):
+        say("stage cache: process %d" % k)
+        r = subprocess.run([sys.executable, "-c", _CACHE_SCRIPT, H.ROOT], env=env, capture_output=True, text=True,
+                           timeout=240)
+        assert r.returncode == 0, r.stderr[-2000:]
+        logs = [l for l in r.stderr.splitlines() if l.startswith("[qr jit]")]
+        say("stage cache: %s" % logs)
+        digest = [l for l in r.stdout.splitlines() if l.startswith("digest")]
+        runs.append((logs, digest))
+    first, second = runs
+    assert first[0] and " 0 kernels from the disk cache" in first[0][0]
+    assert second[0] and all(l.endswith(", 0 to compile") for l in second[0])
+    assert first[1] == second[1] and first[1]
+
+
 def test_norm_and_expectation():
     rng = np.random.default_rng(77)
     n = 12
