@@ -60,6 +60,8 @@ def parse():
     ap.add_argument("--kernels", action="store_true",
                     help="also time each standalone kernel (reference-order gates, custom tables, sampler, "
                          "reductions, streaming ceiling)")
+    ap.add_argument("--no-resample", action="store_true",
+                    help="skip the measure_all_without_cache (batched re-simulation) leg")
     ap.add_argument("--capacity", action="store_true",
                     help="also run the 33-qubit complex128 and 34-qubit complex64 capacity circuits")
     return ap.parse_args()
@@ -457,6 +459,41 @@ def reference_algorithm_baseline(n_target, depth):
                     % n_target}
 
 
+def bench_resample(args):
+    """measure_all_without_cache's per-shot re-simulation: 500 shots of a 12-qubit brickwork
+    (depth 10) whose 2-qubit custom gate gets a new table every shot, batched
+    (qr_sample_programs) against one qr_apply_program + qr_sample per shot; the bins must be
+    identical."""
+    n, shots = 12, 500
+    base = W.brickwork(n, depth=10, seed=5)
+    mid = len(base) // 2
+    gates = base[:mid] + [("Custom", 3, (7,), 0.0, 0)] + base[mid:]
+    progs = [W.to_program(gates, {mid: (np.ascontiguousarray(W.random_unitary(2, 1000 + s).T), np.ones(4, np.uint8))})
+             .c_struct() for s in range(shots)]  # the tabulated programs, built before the clock
+    u = np.array(W.uniforms(shots, 555))
+    _capi.sample_programs(n, progs[:2], [1, 1], u[:2])  # plan and stages
+    t0 = time.perf_counter()
+    got = _capi.sample_programs(n, progs, [1] * shots, u)
+    t_batch = time.perf_counter() - t0
+    st = _capi.State(n)
+    st.set_options(fused=1, jit=1)
+    st.apply(progs[0])
+    want = np.empty(shots, dtype=np.uint64)
+    t0 = time.perf_counter()
+    for j in range(shots):
+        st.reset()
+        st.apply(progs[j])
+        want[j] = st.sample(u[j:j + 1])[0]
+    t_serial = time.perf_counter() - t0
+    st.close()
+    same = bool(np.array_equal(got, want))
+    log("bench: measure_all_without_cache %d shots at %d qubits: batched %.1f ms, per shot %.1f ms (%.0fx), "
+        "identical %s" % (shots, n, t_batch * 1e3, t_serial * 1e3, t_serial / t_batch, same))
+    return {"shots": shots, "qubits": n, "gates": len(gates), "batched_ms": round(t_batch * 1e3, 2),
+            "per_shot_ms": round(t_serial * 1e3, 2), "speedup": round(t_serial / t_batch, 1),
+            "identical_bins": same}
+
+
 def kernel_legs(args):
     """Each standalone kernel on a 30-qubit state (complex128 unless noted), timed with HIP events
     on the state's stream (qr_profile), for per-kernel HBM GB/s: reference-order gates (1, 2 and 3
@@ -578,6 +615,8 @@ def main():
         out["precision"] = precision_report(args, ms)
     if args.capacity:
         out["capacity"] = capacity(args)
+    if not args.no_resample:
+        out["measure_all_without_cache"] = bench_resample(args)
     if args.kernels:
         out["kernels"] = kernel_legs(args)
     if not args.no_cpu_baseline:

@@ -582,7 +582,8 @@ class SimulatedCircuit {
   }
   // measure_all_without_cache (simulated_circuit.rs:81-114): the first shot from the current
   // register, every later shot from the circuit re-simulated on |0...0> with its custom
-  // functions called again (2^k calls per custom gate and shot, where the reference makes 2^n).
+  // functions called again (2^k calls per custom gate and shot, where the reference makes 2^n),
+  // all re-simulations batched on the device. The simulated state is left as it was.
   Measurement<Bins> measure_all_without_cache(size_t shots);
 
   Measurement<const states::SuperPosition*> get_state() const {
@@ -626,8 +627,8 @@ class SimulatedCircuit {
     detail::check(qr_state_get_amplitudes(dev_->st, reinterpret_cast<double*>(a.data()), a.size()));
     host_ = states::SuperPosition::new_with_amplitudes_unchecked(a);
   }
-  void bin_draws(const std::vector<double>& u, Bins& bins) const {
-    const std::vector<uint64_t> idx = sample_indices(u);
+  void bin_draws(const std::vector<double>& u, Bins& bins) const { bin_indices(sample_indices(u), bins); }
+  void bin_indices(const std::vector<uint64_t>& idx, Bins& bins) const {
     for (uint64_t i : idx) {  // add_to_bin (simulated_circuit.rs:116-130)
       if (i == UINT64_MAX) {
         if (!disable_warnings_)
@@ -904,27 +905,36 @@ inline Measurement<SimulatedCircuit::Bins> SimulatedCircuit::measure_all_without
   bin_draws({detail::rng().f64()}, bins);
   if (config_progress_) std::printf("Measured state # 1/%zu\n", shots);
   // Every later shot re-simulates the circuit on |0...0>, calling the custom functions again
-  // (simulated_circuit.rs:91-111). Draws taken while the re-tabulated program stays equal to
-  // the one on the device are sampled from that state together.
-  std::vector<double> pending;
+  // (simulated_circuit.rs:91-111). The shots' programs are tabulated here in shot order (one
+  // program per run of equal consecutive tables) and simulated and measured together on the
+  // device (qr_sample_programs): programs of one structure run each sweep as one launch.
+  constexpr size_t kPrograms = 1024;  // programs per device call
+  std::vector<std::shared_ptr<detail::Program>> progs;
+  std::vector<uint64_t> counts;
+  std::vector<double> draws;
   auto flush = [&]() {
-    if (!pending.empty()) bin_draws(pending, bins);
-    pending.clear();
+    if (progs.empty()) return;
+    std::vector<qr_program> views;
+    views.reserve(progs.size());
+    for (const auto& p : progs) views.push_back(p->view());
+    std::vector<uint64_t> idx(draws.size());
+    detail::check(qr_sample_programs((uint32_t)num_qubits_, precision_, views.data(), views.size(), counts.data(),
+                                     draws.data(), idx.data()));
+    bin_indices(idx, bins);
+    progs.clear();
+    counts.clear();
+    draws.clear();
   };
-  std::shared_ptr<detail::Program> on_device;
   for (size_t i = 0; i + 1 < shots; ++i) {
     auto prog = Circuit::lower_gates(gates_, num_qubits_);
-    if (!on_device || !(*prog == *on_device)) {
-      flush();
-      detail::check(qr_state_reset(dev_->st));
-      qr_program v = prog->view();
-      detail::check(qr_apply_program(dev_->st, &v));
-      host_.reset();
-      on_device = prog;
-      program_ = prog;
+    if (progs.empty() || !(*prog == *progs.back())) {
+      if (progs.size() >= kPrograms) flush();
+      progs.push_back(std::move(prog));
+      counts.push_back(0);
     }
+    ++counts.back();
     if (config_progress_) std::printf("Register reset to zero state\n");
-    pending.push_back(detail::rng().f64());
+    draws.push_back(detail::rng().f64());
     if (config_progress_) std::printf("Measured state # %zu/%zu\n", i + 2, shots);
   }
   flush();

@@ -219,6 +219,18 @@ qr_status qr_apply_program(qr_state* state, const qr_program* program);
  * injected instead of fastrand::f64(). Used by measure_all (simulated_circuit.rs:63-130). */
 qr_status qr_sample(qr_state* state, const double* uniforms, uint64_t shots, uint64_t* idx_out);
 
+/* Many programs, each simulated from |0...0> and measured with its own draws: program j takes
+ * draws_per_program[j] uniforms (consecutive in `uniforms`, programs in order) and idx_out
+ * receives, per draw, what qr_sample returns on program j's state. The per-shot loop of
+ * SimulatedCircuit::measure_all_without_cache (simulated_circuit.rs:81-114), which
+ * re-simulates the circuit for every shot because custom functions may answer differently
+ * (examples/post_select.rs:39-49), done for all shots at once: programs of one structure share
+ * a plan and their sweeps run as one launch over all of their states. Fused mode; states use
+ * the current device and are freed before returning. */
+qr_status qr_sample_programs(uint32_t num_qubits, qr_precision precision, const qr_program* programs,
+                             uint64_t count, const uint64_t* draws_per_program, const double* uniforms,
+                             uint64_t* idx_out);
+
 /* sum_i |a_i|^2 in f64. */
 qr_status qr_state_norm_sqr(qr_state* state, double* out);
 

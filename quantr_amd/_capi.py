@@ -83,6 +83,8 @@ _SIGNATURES = [
     ("qr_validate_program", C.c_int, [C.c_uint32, C.POINTER(QrProgram)]),
     ("qr_apply_program", C.c_int, [C.c_void_p, C.POINTER(QrProgram)]),
     ("qr_sample", C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]),
+    ("qr_sample_programs", C.c_int, [C.c_uint32, C.c_int, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+                                     C.c_void_p]),
     ("qr_state_norm_sqr", C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
     ("qr_expectation", C.c_int, [C.c_void_p, C.POINTER(QrObservable), C.POINTER(C.c_double)]),
     ("qr_expectation_noalloc", C.c_int, [C.c_void_p, C.POINTER(QrObservable), C.POINTER(C.c_double)]),
@@ -229,6 +231,21 @@ class Observable:
                          p.ctypes.data if len(p) else None, len(w))
         o._keep = (t, w, p)
         return o
+
+
+def sample_programs(n, programs, draws, uniforms, precision=QR_C128):
+    """qr_sample_programs: programs[j] simulated from |0...0> and measured with draws[j]
+    consecutive uniforms; returns the indices (UINT64_MAX = None), draws in order."""
+    structs = [p.c_struct() if isinstance(p, Program) else p for p in programs]
+    arr = (QrProgram * max(1, len(structs)))(*structs)
+    d = np.ascontiguousarray(np.asarray(draws, dtype=np.uint64))
+    u = np.ascontiguousarray(np.asarray(uniforms, dtype=np.float64))
+    if int(d.sum()) != u.size:
+        raise ValueError("draws must add up to the number of uniforms")
+    out = np.empty(u.size, dtype=np.uint64)
+    check(lib().qr_sample_programs(n, precision, C.cast(arr, C.c_void_p), len(structs), d.ctypes.data,
+                                   u.ctypes.data, out.ctypes.data))
+    return out
 
 
 class State:

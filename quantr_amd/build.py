@@ -24,7 +24,7 @@ HIP_FLAGS = [
     "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-I" + INCLUDE, "-I" + CSRC,
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["kernels.hip", "tile_interp.hip", "lower.cpp", "planner.cpp", "runtime.cpp", "batch.cpp", "capi.cpp", "jit.cpp", "sampler.cpp", "noalloc.cpp"]
+SOURCES = ["kernels.hip", "tile_interp.hip", "lower.cpp", "planner.cpp", "runtime.cpp", "batch.cpp", "capi.cpp", "jit.cpp", "sampler.cpp", "noalloc.cpp", "shots.cpp"]
 
 
 def embed_tile_ops():

@@ -208,6 +208,15 @@ qr_status qr_state_last_plan(const qr_state* st, qr_plan_stats* out) {
   });
 }
 
+qr_status qr_sample_programs(uint32_t num_qubits, qr_precision precision, const qr_program* programs,
+                             uint64_t count, const uint64_t* draws_per_program, const double* uniforms,
+                             uint64_t* idx_out) {
+  return guard([&] {
+    if (precision != QR_C128 && precision != QR_C64) throw Error{QR_ERR_ARG, "unknown precision"};
+    sample_programs(num_qubits, precision == QR_C128 ? 0 : 1, programs, count, draws_per_program, uniforms, idx_out);
+  });
+}
+
 qr_status qr_sample(qr_state* st, const double* uniforms, uint64_t shots, uint64_t* idx_out) {
   return guard([&] {
     if (!st) throw Error{QR_ERR_ARG, "state is NULL"};

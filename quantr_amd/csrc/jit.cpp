@@ -126,8 +126,11 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
   if (dual) o << C << "* __restrict__ lam, ";
   o << "const double2* __restrict__ cf, uint64_t nontile, uint64_t ntiles";
   if (dual) o << ", double* __restrict__ part, uint64_t nslots";
-  else o << ", uint32_t order";
+  else o << ", uint32_t order, uint64_t inst_stride, uint64_t cf_stride";
   o << ") {\n";
+  // several states of one structure in one launch: instance blockIdx.y has its amplitudes and
+  // coefficients at fixed strides (zero for a single state)
+  if (!dual) o << "  psi += (uint64_t)blockIdx.y * inst_stride;\n  cf += (uint64_t)blockIdx.y * cf_stride;\n";
   o << "  using R = " << R << ";\n";
   o << "  __shared__ " << C << " lds[" << (split ? (1 << (K - 1)) : (lds2 ? 2 : 1) * (1 << K)) << "];\n";
   o << "  const uint32_t tid = threadIdx.x;\n";
@@ -748,9 +751,11 @@ hipFunction_t jit_function(const std::string& source) {
 }
 
 hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* psi, const double2* coefs,
-                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s, uint32_t order) {
+                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s, uint32_t order, uint32_t instances,
+                      uint64_t inst_stride, uint64_t cf_stride) {
   if (!fn) return hipErrorInvalidValue;
   if (ntiles == 0 || ntiles > 0x7fffffffull) return hipErrorInvalidValue;
+  if (instances == 0 || instances > 65535 || (instances > 1 && persistent)) return hipErrorInvalidValue;
   unsigned grid = (unsigned)ntiles;
   const unsigned threads = 1u << (K - RB);
   if (persistent) {
@@ -763,8 +768,8 @@ hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* ps
     grid = (unsigned)std::min<uint64_t>(ntiles, (uint64_t)cus * per_cu);
   }
   if (persistent) order = 0;
-  void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles, &order};
-  return hipModuleLaunchKernel(fn, grid, 1, 1, threads, 1, 1, 0, s, args, nullptr);
+  void* args[] = {&psi, (void*)&coefs, &nontile_mask, &ntiles, &order, &inst_stride, &cf_stride};
+  return hipModuleLaunchKernel(fn, grid, instances, 1, threads, 1, 1, 0, s, args, nullptr);
 }
 
 hipError_t jit_launch_dual(hipFunction_t fn, int K, int RB, void* psi, void* lam, const double2* coefs,

@@ -23,9 +23,12 @@ void jit_prepare_without_spills(std::vector<std::string>& sources);
 hipFunction_t jit_function(const std::string& source);
 // Launches a prepared stage: one workgroup per tile. `order` 0 takes tiles in index order;
 // 1 gives each XCD a contiguous eighth of them (non-persistent stages, tile counts divisible
-// by 8; otherwise index order). Results do not depend on it.
+// by 8; otherwise index order). Results do not depend on it. With `instances` > 1
+// (non-persistent stages), instance y's state starts inst_stride amplitudes and its
+// coefficients cf_stride double2 after instance 0's (grid y = instance, at most 65535).
 hipError_t jit_launch(hipFunction_t fn, int K, int RB, bool persistent, void* psi, const double2* coefs,
-                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s, uint32_t order = 0);
+                      uint64_t ntiles, uint64_t nontile_mask, hipStream_t s, uint32_t order = 0,
+                      uint32_t instances = 1, uint64_t inst_stride = 0, uint64_t cf_stride = 0);
 // Launches a prepared dual stage; gradient taps write part[param * nslots + slot].
 hipError_t jit_launch_dual(hipFunction_t fn, int K, int RB, void* psi, void* lam, const double2* coefs,
                            uint64_t ntiles, uint64_t nontile_mask, double* part, uint64_t nslots, hipStream_t s);

@@ -603,6 +603,61 @@ __global__ __launch_bounds__(256) void k_pauli_sum(typename CT<R>::T* __restrict
   }
 }
 
+// |0...0> in each of `count` states of dim amplitudes laid out back to back.
+template <typename R>
+__global__ __launch_bounds__(256) void k_init_instances(typename CT<R>::T* __restrict__ psi, uint64_t dim,
+                                                        uint64_t total) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += stride) {
+    psi[i].x = (R)((i & (dim - 1)) == 0 ? 1 : 0);
+    psi[i].y = (R)0;
+  }
+}
+
+// One workgroup per state (states back to back, `stride` amplitudes apart): its draws
+// u[off[y] .. off[y+1]) (ascending) placed by the left fold of its probabilities in index
+// order, as k_sample_fold does for one state: probabilities in parallel, the fold in one lane.
+template <typename R>
+__global__ __launch_bounds__(256) void k_sample_fold_instances(const typename CT<R>::T* __restrict__ psi,
+                                                               uint64_t dim, uint64_t stride,
+                                                               const double* __restrict__ u,
+                                                               const uint64_t* __restrict__ off,
+                                                               uint64_t* __restrict__ out) {
+#pragma clang fp contract(off)
+  constexpr int CH = 2048;
+  __shared__ double p[CH];
+  __shared__ int done;
+  const typename CT<R>::T* ps = psi + (uint64_t)blockIdx.x * stride;
+  const uint64_t d1 = off[blockIdx.x + 1];
+  uint64_t d = off[blockIdx.x];
+  double cum = 0.0;
+  if (threadIdx.x == 0) done = (d >= d1);
+  __syncthreads();
+  for (uint64_t base = 0; base < dim && !done; base += CH) {
+    for (int q = threadIdx.x; q < CH; q += blockDim.x) {
+      const uint64_t i = base + q;
+      double v = 0.0;
+      if (i < dim) {
+        const double re = (double)ps[i].x, im = (double)ps[i].y;
+        v = re * re + im * im;
+      }
+      p[q] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const uint64_t cnt = (dim - base < (uint64_t)CH) ? (dim - base) : (uint64_t)CH;
+      for (uint64_t q = 0; q < cnt; ++q) {
+        cum = cum + p[q];
+        while (d < d1 && u[d] < cum) out[d++] = base + q;
+      }
+      done = (d >= d1);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    while (d < d1) out[d++] = ~0ull;
+}
+
 // Partial sums of Re <psi|O|psi> = Re sum_o conj(psi_o) (O psi)_o in one pass, one per workgroup.
 template <typename R>
 __global__ __launch_bounds__(256) void k_pauli_expect(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
@@ -1022,6 +1077,26 @@ hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const Pa
   else
     dev::k_pauli_expect<float>
         <<<nblocks, 256, 0, s>>>((const float2*)psi, dim, groups, ngroups, terms, part, accumulate);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_instances(int prec, void* psi, uint64_t dim, uint64_t count, hipStream_t s) {
+  const uint64_t total = dim * count;
+  if (prec == 0)
+    dev::k_init_instances<double><<<grid_for(total, 256, 1u << 16), 256, 0, s>>>((double2*)psi, dim, total);
+  else
+    dev::k_init_instances<float><<<grid_for(total, 256, 1u << 16), 256, 0, s>>>((float2*)psi, dim, total);
+  return hipGetLastError();
+}
+
+hipError_t launch_sample_fold_instances(int prec, const void* psi, uint64_t dim, uint64_t count, const double* u,
+                                        const uint64_t* off, uint64_t* out, hipStream_t s) {
+  if (count == 0) return hipSuccess;
+  if (count > 0x7fffffffull) return hipErrorInvalidValue;
+  if (prec == 0)
+    dev::k_sample_fold_instances<double><<<(unsigned)count, 256, 0, s>>>((const double2*)psi, dim, dim, u, off, out);
+  else
+    dev::k_sample_fold_instances<float><<<(unsigned)count, 256, 0, s>>>((const float2*)psi, dim, dim, u, off, out);
   return hipGetLastError();
 }
 

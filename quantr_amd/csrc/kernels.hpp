@@ -127,6 +127,13 @@ struct PauliGroup {
 // (the whole observable in one pass).
 hipError_t launch_pauli_sum(int prec, void* lam, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,
                             const PauliZTerm* terms, hipStream_t s);
+// |0...0> in each of `count` states of dim amplitudes stored back to back.
+hipError_t launch_init_instances(int prec, void* psi, uint64_t dim, uint64_t count, hipStream_t s);
+// For each of `count` states stored back to back: its draws u[off[y] .. off[y+1]) (ascending)
+// placed by the left fold of its probabilities in index order (out = first index whose
+// cumulative probability exceeds the draw, UINT64_MAX if none).
+hipError_t launch_sample_fold_instances(int prec, const void* psi, uint64_t dim, uint64_t count, const double* u,
+                                        const uint64_t* off, uint64_t* out, hipStream_t s);
 // part[b] = partial sums of Re <psi| O |psi> over nblocks blocks, O as in launch_pauli_sum
 // (added to part[b] with `accumulate`). Every group costs a read of the state.
 hipError_t launch_pauli_expect(int prec, const void* psi, uint64_t dim, const PauliGroup* groups, int ngroups,

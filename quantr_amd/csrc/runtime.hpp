@@ -186,6 +186,10 @@ double sum_partials_host(const double* v, size_t count);
 // measure_all's sampling (sampler.cpp): draws `us` in ascending order; res[s] = the first index
 // whose left-fold cumulative probability exceeds us[s], or UINT64_MAX (None).
 void sample_exact_parallel(qr_state* st, const std::vector<double>& us, std::vector<uint64_t>& res);
+// measure_all_without_cache's re-simulation, batched (shots.cpp): programs[j] simulated from
+// |0...0>, then its draws[j] uniforms (consecutive in u) placed by the left fold into out.
+void sample_programs(uint32_t n, int prec, const qr_program* programs, uint64_t count, const uint64_t* draws,
+                     const double* u, uint64_t* out);
 double norm_sqr(qr_state* st);
 double expectation(qr_state* st, const std::vector<PauliMasks>& terms);
 // The same value through fixed device and host arrays: no allocation (noalloc.cpp).

@@ -322,6 +322,45 @@ def test_measured_plans_25q():
     assert np.array_equal(first, again)
 
 
+@pytest.mark.parametrize("precision", [_capi.QR_C128, _capi.QR_C64])
+def test_sample_programs_batched(precision):
+    """measure_all_without_cache's batched re-simulation (qr_sample_programs): every program's
+    draws equal what qr_sample gives on that program's own state, bit for bit. Two structures
+    interleaved (one with non-unitary custom tables, whose "None" rows leave states with
+    norm below one), programs with no draws, and sizes from one tile to many."""
+    rng = np.random.default_rng(880 + precision)
+    for n in (6, 13, 17):
+        base_a, cust_a = H.random_gates(n, 40, rng, with_custom=True)
+        base_b, _ = H.random_gates(n, 25, rng)
+        progs, draws, us = [], [], []
+        for j in range(24):
+            if j % 3 == 2:  # structure B: new angles
+                gates = [(k, pos, ctl, float(rng.uniform(-3, 3)), ai) for (k, pos, ctl, _, ai) in base_b]
+                p = H.to_program(gates)
+            else:  # structure A: new custom tables of the same shape, some with None rows
+                cust = {}
+                for i, (imgs, some) in cust_a.items():
+                    im = rng.standard_normal(imgs.shape) + 1j * rng.standard_normal(imgs.shape)
+                    im /= np.linalg.norm(im, axis=1, keepdims=True)
+                    cust[i] = (im, some)
+                p = H.to_program(base_a, cust)
+            k = int(rng.integers(0, 6))
+            progs.append(p)
+            draws.append(k)
+            us.extend(rng.random(k).tolist())
+        say("sample programs n=%d prec=%d: %d programs, %d draws" % (n, precision, len(progs), len(us)))
+        got = _capi.sample_programs(n, progs, draws, us, precision)
+        at = 0
+        for p, k in zip(progs, draws):
+            st = _capi.State(n, precision)
+            st.set_options(fused=1, jit=1)  # the same compiled stages as the batched run
+            st.apply(p)
+            want = st.sample(np.array(us[at:at + k])) if k else np.zeros(0, dtype=np.uint64)
+            st.close()
+            assert np.array_equal(got[at:at + k], want), (n, at)
+            at += k
+
+
 _CACHE_SCRIPT = r"""
 import hashlib, sys
 sys.path.insert(0, sys.argv[1])
