@@ -162,6 +162,30 @@ def test_identity_layout_knob(monkeypatch):
     assert np.max(np.abs(got - H.oracle_simulate(n, p))) < TOL_FUSED
 
 
+@pytest.mark.parametrize("diag", [(0.0, np.sqrt(2.0)), (np.sqrt(2.0), 0.0), (0.0, 0.0), (1e-300, 1.0)])
+@pytest.mark.parametrize("controls", [(), (0,)])
+def test_custom_diagonal_zero_entries_gpu(diag, controls):
+    """Custom diagonal gates with zero or tiny entries (post-selection onto |1>, the zero gate)
+    in fused tile stages, interpreted and compiled: finite and equal to the oracle."""
+    n = 3
+    dim = 1 << (1 + len(controls))
+    images = np.eye(dim, dtype=np.complex128)
+    images[dim - 2, dim - 2] = diag[0]
+    images[dim - 1, dim - 1] = diag[1]
+    p = H.Program()
+    for w in range(n):
+        p.add("H", w)
+    p.add_custom(1, list(controls), images, np.ones(dim, dtype=np.uint8))
+    p.add("Rx", 2, (), 0.3)
+    p.add("T", 1)
+    want = H.oracle_simulate(n, p)
+    for tile_bits in (2, 3):
+        for jit in (0, 1):
+            got = run_gpu(n, p, fused=1, tile_bits=tile_bits, jit=jit)
+            assert np.all(np.isfinite(got)), (diag, controls, tile_bits, jit)
+            assert np.max(np.abs(got - want)) < TOL_FUSED, (diag, controls, tile_bits, jit)
+
+
 def test_complex64_random():
     rng = np.random.default_rng(404)
     for trial in range(10):
