@@ -185,6 +185,7 @@ struct PlanParams {
   int lds_split = 0;  // remaps exchange the tile through LDS in two halves (half the LDS)
   int dyn_low = 0;    // moving layout: stages choose which bits the physical low bits hold
   uint32_t seed = 0;  // 0: ties between equally good bits go to the lowest; else a seeded pick
+  int tie_slack = 0;  // with a seed: bits up to tie_slack ops short of the best also tie
 };
 
 // A remap done in two halves ("split", TileOpHost::flags bit 0) moves the amplitudes whose
@@ -217,7 +218,7 @@ std::vector<Stage> plan_fused_balanced(uint32_t n, const std::vector<FastOp>& op
 // one plan (plan_fused's) where plan_fused_balanced would not balance.
 std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vector<FastOp>& ops,
                                                      const PlanParams& pp, int bytes_per_amp, bool dual,
-                                                     int count);
+                                                     int count, std::vector<PlanParams>* chosen = nullptr);
 // Appends what the planner's output depends on for one op (kind, bits, controls and the
 // value-derived forms), so that programs with equal keys share one plan; coefficients are
 // then taken from each program's own ops (TileOpHost::src).

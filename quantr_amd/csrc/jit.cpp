@@ -743,6 +743,18 @@ void jit_prepare_without_spills(std::vector<std::string>& sources) {
   for (size_t j = 0; j < redo.size(); ++j) sources[redo[j]] = std::move(plain[j]);
 }
 
+std::string jit_cache_dir() { return cache_dir(); }
+
+std::string cache_key_hex(const std::string& bytes) {
+  Hash128 h;
+  h.add(bytes);
+  return h.hex();
+}
+
+bool cache_write(const std::string& path, const std::string& text) {
+  return write_file_atomic(path, text.data(), text.size());
+}
+
 hipFunction_t jit_function(const std::string& source) {
   std::lock_guard<std::mutex> lk(g_mu);
   auto it = g_cache.find(source);

@@ -34,5 +34,10 @@ hipError_t jit_launch_dual(hipFunction_t fn, int K, int RB, void* psi, void* lam
                            uint64_t ntiles, uint64_t nontile_mask, double* part, uint64_t nslots, hipStream_t s);
 uint64_t jit_dual_slots(int n, int K, int RB);
 size_t jit_cache_size();
+// The on-disk cache directory ("" when disabled or unusable), a 128-bit hex hash of some bytes,
+// and an atomic file write (temporary name, then rename) for entries kept there.
+std::string jit_cache_dir();
+std::string cache_key_hex(const std::string& bytes);
+bool cache_write(const std::string& path, const std::string& text);
 
 }  // namespace qr

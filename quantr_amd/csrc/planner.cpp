@@ -203,20 +203,27 @@ uint64_t mix64(uint64_t z) {
 
 // Greedy choice of `want` extra bits from `cands` on top of `base` maximising absorption.
 // Ties go to the first candidate, or with a nonzero `salt` to the one whose salted hash is
-// lowest (another plan of the same quality for the same program).
+// lowest (another plan of the same quality for the same program); with `slack`, bits that
+// absorb up to `slack` ops fewer than the best also tie (plans greedy would not make).
 uint64_t greedy_bits(uint64_t base, int want, const std::vector<int>& cands,
                      const std::vector<int>& remaining, const std::vector<Masks>& mk,
-                     const std::vector<uint8_t>& eligible, uint64_t salt = 0) {
+                     const std::vector<uint8_t>& eligible, uint64_t salt = 0, int slack = 0) {
   uint64_t cur = base;
+  std::vector<std::pair<int, int>> score;
   for (int step = 0; step < want; ++step) {
     int best = -1, best_bit = -1;
+    score.clear();
     for (int b : cands) {
       if (cur & (1ull << b)) continue;
       const int c = count_absorb(remaining, mk, eligible, cur | (1ull << b));
-      if (c > best || (salt && c == best && mix64(salt ^ (uint64_t)b) < mix64(salt ^ (uint64_t)best_bit))) {
-        best = c;
-        best_bit = b;
-      }
+      score.push_back({b, c});
+      best = std::max(best, c);
+    }
+    const int lowest = salt ? best - slack : best;
+    const uint64_t h = salt ^ ((uint64_t)step << 40);
+    for (const auto& bc : score) {
+      if (bc.second < lowest) continue;
+      if (best_bit < 0 || (salt && mix64(h ^ (uint64_t)bc.first) < mix64(h ^ (uint64_t)best_bit))) best_bit = bc.first;
     }
     if (best_bit < 0) break;
     cur |= 1ull << best_bit;
@@ -433,7 +440,8 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
         if (phys[b] != (int)b || (int)b < c) B |= 1ull << b;
     int have = __builtin_popcountll(B);
     const uint64_t salt = pp.seed ? mix64(((uint64_t)pp.seed << 32) ^ stages.size()) | 1 : 0;
-    if (have < k && !restore_only) B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible, salt);
+    if (have < k && !restore_only)
+      B = greedy_bits(B, k - have, all_bits, remaining, mk, eligible, salt, pp.seed ? pp.tie_slack : 0);
     // fill up if greedy ran out of useful bits (keeps k fixed for the kernel variant)
     for (uint32_t b = 0; b < n && __builtin_popcountll(B) < k; ++b) B |= 1ull << b;
     if (dyn && !restore_only && !identity_layout()) {
@@ -1009,36 +1017,56 @@ double plan_cost(const std::vector<Stage>& st, int bytes_per_amp) {
 
 std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vector<FastOp>& ops,
                                                      const PlanParams& pp, int bytes_per_amp, bool dual,
-                                                     int count) {
+                                                     int count, std::vector<PlanParams>* chosen) {
   std::vector<std::vector<Stage>> out;
   if (pp.max_ops > 0 || dual || n < kBalanceMinQubits || pp.tile_bits >= (int)n) {
     out.push_back(plan_fused(n, ops, pp, bytes_per_amp, dual));
+    if (chosen) chosen->assign(1, pp);
     return out;
   }
-  // every cap with the first-candidate tie rule, and the caps that balance best with seeded
-  // tie rules: greedy choices that tie lead to plans of different sweep counts (on the
-  // 30-qubit brickwork, 23 to 28 sweeps over seeds), so a few seeds are planned as well
+  // every cap with the first-candidate tie rule; the caps that balance best with seeded tie
+  // rules (greedy choices that tie lead to plans of different sweep counts: 23 to 28 for the
+  // 30-qubit brickwork over seeds); and, for large states, seeds that also let near-ties
+  // (one op short of the best) tie, whose rare short plans pay for their planning time
+  // (the 30-qubit brickwork in 20 sweeps, against 23)
   static const int caps[] = {0, 160, 140, 120, 100, 90, 80};
   static const int seeded_caps[] = {0, 160, 140, 120, 100};
-  constexpr int kSeeds = 8;
-  std::vector<std::pair<int, uint32_t>> jobs;
-  for (int cap : caps) jobs.push_back({cap, 0u});
+  static const int slack_caps[] = {0, 160};
+  constexpr uint32_t kSeeds = 8;
+  const uint32_t slack_seeds = n >= 28 ? 64 : 16;
+  std::vector<PlanParams> jobs;
+  for (int cap : caps) {
+    PlanParams q = pp;
+    q.max_ops = cap;
+    jobs.push_back(q);
+  }
   for (uint32_t seed = 1; seed < kSeeds; ++seed)
-    for (int cap : seeded_caps) jobs.push_back({cap, seed});
+    for (int cap : seeded_caps) {
+      PlanParams q = pp;
+      q.max_ops = cap;
+      q.seed = seed;
+      jobs.push_back(q);
+    }
+  for (uint32_t seed = 1; seed <= slack_seeds; ++seed)
+    for (int cap : slack_caps) {
+      PlanParams q = pp;
+      q.max_ops = cap;
+      q.seed = seed;
+      q.tie_slack = 1;
+      jobs.push_back(q);
+    }
   const int nc = (int)jobs.size();
   std::vector<std::vector<Stage>> plans(nc);
   std::vector<std::exception_ptr> errs(nc);
   std::atomic<int> next{0};
-  const int nthreads = std::max(1, std::min<int>(nc, (int)std::thread::hardware_concurrency()));
+  // at most 16 planning threads (a host share per GPU), fewer on smaller hosts
+  const int nthreads = std::max(1, std::min<int>({nc, 16, (int)std::thread::hardware_concurrency()}));
   std::vector<std::thread> th;
   for (int t = 0; t < nthreads; ++t)
     th.emplace_back([&] {
       for (int i = next++; i < nc; i = next++) {
         try {
-          PlanParams q = pp;
-          q.max_ops = jobs[i].first;
-          q.seed = jobs[i].second;
-          plans[i] = plan_fused(n, ops, q, bytes_per_amp, dual);
+          plans[i] = plan_fused(n, ops, jobs[i], bytes_per_amp, dual);
         } catch (...) {
           errs[i] = std::current_exception();
         }
@@ -1062,12 +1090,14 @@ std::vector<std::vector<Stage>> plan_fused_candidates(uint32_t n, const std::vec
     return g;
   };
   std::vector<std::vector<int>> seen;
+  if (chosen) chosen->clear();
   for (int i : idx) {
     if ((int)out.size() >= std::max(1, count)) break;
     std::vector<int> sh = shape(plans[i]);
     if (std::find(seen.begin(), seen.end(), sh) != seen.end()) continue;
     seen.push_back(std::move(sh));
     out.push_back(std::move(plans[i]));
+    if (chosen) chosen->push_back(jobs[i]);
   }
   return out;
 }

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <fstream>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -211,6 +212,35 @@ void launch_standalone(qr_state* st, void* amp, const FastOp& op, const PackedPl
 // Plans depend on the program's structure only (plan_key_append); parameter sweeps, gradient
 // loops and serving re-apply one structure with new coefficients, so plans and their
 // compiled stages are kept per process (least recently used dropped beyond 64).
+// The balanced planner's search (tens of plans) is remembered on disk beside the compiled
+// stages: the winning cap and tie rule per structure, so a later process plans once.
+static std::vector<Stage> plan_balanced_cached(uint32_t n, const std::vector<FastOp>& ops, const PlanParams& pp,
+                                               int B, bool dual, const std::vector<uint64_t>& key) {
+  const bool searches = !(pp.max_ops > 0 || dual || n < kBalanceMinQubits || pp.tile_bits >= (int)n);
+  const std::string dir = searches ? jit_cache_dir() : std::string();
+  std::string path;
+  if (!dir.empty()) {
+    std::string bytes((const char*)key.data(), key.size() * sizeof(uint64_t));
+    path = dir + "/plan1-" + cache_key_hex(bytes) + ".txt";
+    std::ifstream f(path);
+    long long cap = -1, seed = -1, slack = -1;
+    if (f >> cap >> seed >> slack && cap >= 0 && cap <= 1000000 && seed >= 0 && seed < (1ll << 32) && slack >= 0 &&
+        slack <= 8) {
+      PlanParams q = pp;
+      q.max_ops = (int)cap;
+      q.seed = (uint32_t)seed;
+      q.tie_slack = (int)slack;
+      return plan_fused(n, ops, q, B, dual);
+    }
+  }
+  std::vector<PlanParams> chosen;
+  std::vector<std::vector<Stage>> c = plan_fused_candidates(n, ops, pp, B, dual, 1, &chosen);
+  if (!path.empty() && !chosen.empty())
+    (void)cache_write(path, std::to_string(chosen[0].max_ops) + " " + std::to_string(chosen[0].seed) + " " +
+                                std::to_string(chosen[0].tie_slack) + "\n");
+  return std::move(c[0]);
+}
+
 static std::mutex g_plan_mu;
 static std::map<std::vector<uint64_t>, std::shared_ptr<CachedPlan>> g_plans;
 static uint64_t g_plan_clock = 0;
@@ -238,7 +268,7 @@ std::shared_ptr<CachedPlan> get_plan_for(uint32_t n, int prec, const std::vector
   auto cp = std::make_shared<CachedPlan>();
   const int B = prec == 0 ? 16 : 8;
   if (fused) {
-    cp->stages = plan_fused_balanced(n, ops, pp, B, dual);
+    cp->stages = plan_balanced_cached(n, ops, pp, B, dual, key);
   } else {
     for (size_t i = 0; i < ops.size(); ++i) {
       Stage s;

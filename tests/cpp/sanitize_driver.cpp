@@ -90,6 +90,8 @@ void run_all_planners(uint32_t n, const Prog& p) {
       for (int split : {0, 1})
         for (int dyn : {0, 1}) {
           PlanParams pp;
+          pp.seed = (uint32_t)(dyn * 7 + split);
+          pp.tie_slack = split;
           pp.tile_bits = std::min<int>(tb, (int)n);
           pp.reg_bits = std::min(4, pp.tile_bits);
           pp.low_bits = lb;

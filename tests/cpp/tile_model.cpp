@@ -277,7 +277,7 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
 // model_run with every plan option: remaps split into two rounds through half the LDS
 // (PlanParams::lds_split) and the moving layout (PlanParams::dyn_low).
 int model_run_opts(uint32_t n, double* state, const qr_program* prog, int tile_bits, int low_bits, int reg_bits,
-                   int max_ops, int lds_split, int dyn_low, uint32_t seed, uint64_t* sweeps_out) {
+                   int max_ops, int lds_split, int dyn_low, uint32_t seed, int tie_slack, uint64_t* sweeps_out) {
   try {
     auto ops = lower_fast(n, lower_exact(n, prog), false);
     PlanParams pp;
@@ -288,6 +288,7 @@ int model_run_opts(uint32_t n, double* state, const qr_program* prog, int tile_b
     pp.lds_split = lds_split;
     pp.dyn_low = dyn_low;
     pp.seed = seed;
+    pp.tie_slack = tie_slack;
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);

@@ -270,16 +270,17 @@ def test_model_custom_diagonal_zero_entries(diag, controls):
         assert np.max(np.abs(got - want)) < 1e-12, (diag, controls, tile_bits)
 
 
-def run_model_opts(n, program, tile_bits, low_bits, reg_bits=4, max_ops=0, lds_split=0, dyn_low=0, seed=0):
+def run_model_opts(n, program, tile_bits, low_bits, reg_bits=4, max_ops=0, lds_split=0, dyn_low=0, seed=0,
+                   tie_slack=0):
     lib = model()
     lib.model_run_opts.argtypes = [C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                   C.c_int, C.c_uint32, C.POINTER(C.c_uint64)]
+                                   C.c_int, C.c_uint32, C.c_int, C.POINTER(C.c_uint64)]
     psi = np.zeros(1 << n, dtype=np.complex128)
     psi[0] = 1.0
     p = program.c_struct()
     sw = C.c_uint64()
     rc = lib.model_run_opts(n, psi.ctypes.data, C.byref(p), tile_bits, low_bits, reg_bits, max_ops, lds_split,
-                            dyn_low, seed, C.byref(sw))
+                            dyn_low, seed, tie_slack, C.byref(sw))
     assert rc == 0, lib.model_error()
     return psi, sw.value
 
@@ -327,10 +328,11 @@ def test_model_moving_layout(tile_bits, low_bits, lds_split):
     assert sweeps <= fixed + 1, (sweeps, fixed)
 
 
-@pytest.mark.parametrize("seed", [1, 5, 7])
-def test_model_seeded_ties(seed):
-    """Seeded tie rules in the greedy bit choice give other plans of the same program (the
-    balanced planner tries several); each must still compute the circuit."""
+@pytest.mark.parametrize("seed,tie_slack", [(1, 0), (5, 0), (7, 0), (3, 1), (29, 1)])
+def test_model_seeded_ties(seed, tie_slack):
+    """Seeded tie rules in the greedy bit choice, near-ties included (tie_slack 1), give other
+    plans of the same program (the balanced planner tries many); each must still compute the
+    circuit."""
     rng = np.random.default_rng(4500 + seed)
     for trial in range(6):
         tile_bits, low_bits = [(12, 3), (10, 4), (13, 4)][trial % 3]
@@ -339,9 +341,10 @@ def test_model_seeded_ties(seed):
         p = H.to_program(gates, customs)
         want = H.oracle_simulate(n, p)
         for max_ops in (0, 40):
-            got, _ = run_model_opts(n, p, tile_bits, low_bits, lds_split=1, dyn_low=1, seed=seed, max_ops=max_ops)
+            got, _ = run_model_opts(n, p, tile_bits, low_bits, lds_split=1, dyn_low=1, seed=seed, max_ops=max_ops,
+                                    tie_slack=tie_slack)
             assert np.max(np.abs(got - want)) < 1e-12, (trial, n, max_ops)
     n = 16
     p = W.to_program(W.brickwork(n, depth=12, seed=seed))
-    got, _ = run_model_opts(n, p, 12, 3, lds_split=1, dyn_low=1, seed=seed)
+    got, _ = run_model_opts(n, p, 12, 3, lds_split=1, dyn_low=1, seed=seed, tie_slack=tie_slack)
     assert np.max(np.abs(got - H.oracle_simulate(n, p))) < 1e-12
