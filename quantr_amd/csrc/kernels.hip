@@ -693,6 +693,40 @@ __global__ __launch_bounds__(256) void k_pauli_expect(const typename CT<R>::T* _
   if (threadIdx.x == 0) part[blockIdx.x] = accumulate ? part[blockIdx.x] + red[0] : red[0];
 }
 
+// f[r] += (-1)^{popcount(r & hz)} a for the positions r of a chunk: one case per pattern hz,
+// so the signs are compile-time constants (adds and subtracts, no selects).
+template <int CH, int HZ>
+__device__ __forceinline__ void add_signed_fixed(double (&f)[CH], double a) {
+#pragma unroll
+  for (int r = 0; r < CH; ++r) {
+    if (__builtin_popcount((unsigned)(r & HZ)) & 1) f[r] -= a;
+    else f[r] += a;
+  }
+}
+
+template <int CH>
+__device__ __forceinline__ void add_signed(double (&f)[CH], double a, uint32_t hz) {
+  static_assert(CH == 16, "chunks of 16 positions");
+  switch (hz) {
+    case 0: add_signed_fixed<CH, 0>(f, a); break;
+    case 1: add_signed_fixed<CH, 1>(f, a); break;
+    case 2: add_signed_fixed<CH, 2>(f, a); break;
+    case 3: add_signed_fixed<CH, 3>(f, a); break;
+    case 4: add_signed_fixed<CH, 4>(f, a); break;
+    case 5: add_signed_fixed<CH, 5>(f, a); break;
+    case 6: add_signed_fixed<CH, 6>(f, a); break;
+    case 7: add_signed_fixed<CH, 7>(f, a); break;
+    case 8: add_signed_fixed<CH, 8>(f, a); break;
+    case 9: add_signed_fixed<CH, 9>(f, a); break;
+    case 10: add_signed_fixed<CH, 10>(f, a); break;
+    case 11: add_signed_fixed<CH, 11>(f, a); break;
+    case 12: add_signed_fixed<CH, 12>(f, a); break;
+    case 13: add_signed_fixed<CH, 13>(f, a); break;
+    case 14: add_signed_fixed<CH, 14>(f, a); break;
+    default: add_signed_fixed<CH, 15>(f, a); break;
+  }
+}
+
 // The same partial sums for groups whose X masks all lie in one tile: tile bits = the set
 // bits of tmask (K of them, the lowest first). Each workgroup walks tiles blockIdx.x,
 // +gridDim.x, ...: it loads the tile into LDS once and takes every partner amplitude
@@ -726,28 +760,21 @@ __global__ __launch_bounds__(256) void k_pauli_expect_tiles(const typename CT<R>
     }
   }
   double acc = 0.0;
-  // the next tile's amplitudes are loaded into registers while this one is evaluated from LDS
-  T nxt[NPT];
-  auto fetch = [&](uint64_t tile) {
-    const uint64_t b = deposit_bits(tile, nontile) | toff;
+  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint64_t base = deposit_bits(t, nontile) | toff;
 #pragma unroll
     for (int r = 0; r < NPT; ++r) {
       uint64_t off = 0;
 #pragma unroll
       for (int q = 0; q < K - 8; ++q)
         if ((r >> q) & 1) off |= hib[q];
-      const T* src = &psi[b | off];
-      nxt[r].x = src->x;
-      nxt[r].y = src->y;
+      const T* src = &psi[base | off];
+      T v;
+      v.x = src->x;
+      v.y = src->y;
+      lds[tid + 256 * r] = v;
     }
-  };
-  if ((uint64_t)blockIdx.x < ntiles) fetch(blockIdx.x);
-  for (uint64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const uint64_t base = deposit_bits(t, nontile) | toff;
-#pragma unroll
-    for (int r = 0; r < NPT; ++r) lds[tid + 256 * r] = nxt[r];
     __syncthreads();
-    if (t + gridDim.x < ntiles) fetch(t + gridDim.x);
     // per group: the coefficient of every position (terms outer, the thread's positions inner),
     // then the partner amplitudes from LDS. The sign (-1)^{popcount(j & zm)}, j = o ^ xm, splits
     // into the thread's part (base ^ xm) and the register part of o: r's bits on the tile bits
@@ -772,19 +799,12 @@ __global__ __launch_bounds__(256) void k_pauli_expect_tiles(const typename CT<R>
 #pragma unroll
           for (int b = 0; b < K - 8; ++b)
             if (zm & hib[b]) hz |= 1u << b;
-          const uint32_t bt = (uint32_t)__popcll(bx & zm) & 1u;
-#pragma unroll
-          for (int r = 0; r < CH; ++r) {
-            const uint32_t neg = bt ^ ((uint32_t)__popc((uint32_t)(c0 + r) & hz) & 1u);
-            fr[r] += neg ? -cr : cr;
-          }
-          if (ci != 0.0) {
-#pragma unroll
-            for (int r = 0; r < CH; ++r) {
-              const uint32_t neg = bt ^ ((uint32_t)__popc((uint32_t)(c0 + r) & hz) & 1u);
-              fi[r] += neg ? -ci : ci;
-            }
-          }
+          // the thread's signed coefficient (its part of the sign, and the chunk's), then each
+          // position adds or subtracts it by the parity of its register bits in hz
+          const uint32_t bt = ((uint32_t)__popcll(bx & zm) ^ (uint32_t)__popc((uint32_t)c0 & hz)) & 1u;
+          const double ar = bt ? -cr : cr;
+          add_signed<CH>(fr, ar, hz & (CH - 1));
+          if (ci != 0.0) add_signed<CH>(fi, bt ? -ci : ci, hz & (CH - 1));
         }
 #pragma unroll
         for (int r = 0; r < CH; ++r) {
