@@ -38,7 +38,15 @@ static int stage_op_limit(const qr_state* st, int tile_bits) {
   return lim;
 }
 
-PlanParams plan_params(const qr_state* st) {
+PlanParams plan_params(const qr_state* st) { return plan_params(st->n, st->prec, st->opt); }
+
+PlanParams plan_params(uint32_t n, int prec, const qr_options& opt) {
+  qr_state shape;  // the fields the tile choices read (no device resources)
+  shape.n = n;
+  shape.prec = prec;
+  shape.dim = 1ull << n;
+  shape.opt = opt;
+  const qr_state* st = &shape;
   PlanParams pp;
   pp.tile_bits = pick_tile_bits(st);
   pp.reg_bits = tile_reg_bits(pp.tile_bits);

@@ -118,6 +118,8 @@ T* upload(qr_state* st, DevBuf& buf, const std::vector<T>& v) {
 
 // ---- plans ----------------------------------------------------------------------------------
 PlanParams plan_params(const qr_state* st);       // forward sweeps
+// The same for a state of n qubits, precision prec (0 = complex128) and options opt.
+PlanParams plan_params(uint32_t n, int prec, const qr_options& opt);
 PlanParams dual_plan_params(const qr_state* st);  // adjoint sweeps (psi and lam per tile)
 
 // Offsets of one stage's data in a packed plan blob.

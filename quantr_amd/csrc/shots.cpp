@@ -71,7 +71,7 @@ void sample_programs(uint32_t n, int prec, const qr_program* programs, uint64_t 
   uint64_t total = 0;
   std::vector<uint64_t> first(count + 1, 0);
   for (uint64_t j = 0; j < count; ++j) {
-    if (draws[j] > (1ull << 40)) throw Error{QR_ERR_ARG, "too many draws for one program"};
+    if (draws[j] > (1ull << 40) - total) throw Error{QR_ERR_ARG, "more than 2^40 draws in all"};
     first[j] = total;
     total += draws[j];
   }
@@ -99,12 +99,9 @@ void sample_programs(uint32_t n, int prec, const qr_program* programs, uint64_t 
   }
   int count_dev = 0;
   if (hipGetDeviceCount(&count_dev) != hipSuccess || count_dev == 0) throw Error{QR_ERR_NO_DEVICE, "no GPU visible"};
-  // planning parameters of a default state of this size and precision
-  qr_state proto;
-  proto.n = n;
-  proto.prec = prec;
-  proto.dim = 1ull << n;
-  const PlanParams pp = plan_params(&proto);
+  // planning parameters of a state of this size and precision with default options
+  const qr_options defaults{1, 0, 0, 2, 0, 0, 2, 0, 0};
+  const PlanParams pp = plan_params(n, prec, defaults);
   const uint64_t dim = 1ull << n;
   const size_t B = prec == 0 ? 16 : 8;
   hipStream_t s = nullptr;
