@@ -601,7 +601,7 @@ def main():
                      "avg_launch_ms": round(avg_ms, 4), "bytes_per_launch": per_launch_bytes,
                      "stream_ceiling": {
                          "kernel": "k_stream_scale (in-place read+write of the same state, one amplitude "
-                                   "per thread)",
+                                   "per thread, non-temporal like the sweeps)",
                          "ms": round(r["stream_ms"], 4),
                          "gbs": round(per_launch_bytes / (r["stream_ms"] * 1e-3) / 1e9, 1),
                          "frac_of_peak": round(per_launch_bytes / (r["stream_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),

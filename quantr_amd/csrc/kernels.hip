@@ -855,12 +855,14 @@ __global__ __launch_bounds__(256) void k_dot_re_partial(const typename CT<R>::T*
 // measured on MI355X).
 template <typename R>
 __global__ __launch_bounds__(256) void k_stream_scale(typename CT<R>::T* __restrict__ psi, uint64_t dim, R f) {
+  // non-temporal, as the tile sweeps of large states access memory
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < dim) {
-    typename CT<R>::T v = psi[i];
+    typename CT<R>::T v;
+    load_nt(v, &psi[i]);
     v.x *= f;
     v.y *= f;
-    psi[i] = v;
+    store_nt(&psi[i], v);
   }
 }
 

@@ -293,6 +293,33 @@ __device__ __forceinline__ void tap_store(double acc, double* row, unsigned long
   if ((tid & (W - 1)) == 0) row[tile * (unsigned long long)(TPB / W) + tid / W] = acc;
 }
 
+// Amplitude loads and stores marked non-temporal (streamed once per sweep, not kept in cache),
+// as one 16- or 8-byte access each.
+typedef double nt_d2 __attribute__((ext_vector_type(2)));
+typedef float nt_f2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void load_nt(double2& d, const double2* p) {
+  const nt_d2 v = __builtin_nontemporal_load((const nt_d2*)p);
+  d.x = v.x;
+  d.y = v.y;
+}
+__device__ __forceinline__ void load_nt(float2& d, const float2* p) {
+  const nt_f2 v = __builtin_nontemporal_load((const nt_f2*)p);
+  d.x = v.x;
+  d.y = v.y;
+}
+__device__ __forceinline__ void store_nt(double2* p, const double2& s) {
+  nt_d2 v;
+  v.x = s.x;
+  v.y = s.y;
+  __builtin_nontemporal_store(v, (nt_d2*)p);
+}
+__device__ __forceinline__ void store_nt(float2* p, const float2& s) {
+  nt_f2 v;
+  v.x = s.x;
+  v.y = s.y;
+  __builtin_nontemporal_store(v, (nt_f2*)p);
+}
+
 // Deposits the bits of `v` into the set bits of `mask`, lowest first (uniform scalar work).
 __device__ __forceinline__ unsigned long long deposit_bits(unsigned long long v, unsigned long long mask) {
   unsigned long long out = 0;
