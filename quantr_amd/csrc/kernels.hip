@@ -28,14 +28,15 @@ __device__ __forceinline__ uint64_t insert_zero_bits(uint64_t x, const uint8_t* 
 // |0...0> (SuperPosition::new_unchecked, super_positions_unchecked.rs:39-46)
 // =====================================================================================
 template <typename R>
-__global__ void k_init_basis(typename CT<R>::T* __restrict__ psi, uint64_t dim, uint64_t one_at) {
+__global__ void k_init_basis(typename CT<R>::T* __restrict__ psi, uint64_t dim, uint64_t one_at, int nt) {
   using C = typename CT<R>::T;
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < dim; i += stride) {
     C v;
     v.x = (i == one_at) ? R(1) : R(0);
     v.y = R(0);
-    psi[i] = v;
+    if (nt) store_nt(&psi[i], v);
+    else psi[i] = v;
   }
 }
 
@@ -336,7 +337,7 @@ __global__ __launch_bounds__(1024) void k_sample_fold(const typename CT<R>::T* _
 // Parallel path, step 1: probability mass of each block of 2^bb amplitudes (f64).
 template <typename R>
 __global__ __launch_bounds__(256) void k_block_mass(const typename CT<R>::T* __restrict__ psi,
-                                                    uint64_t dim, int bb, double* __restrict__ mass) {
+                                                    uint64_t dim, int bb, double* __restrict__ mass, int nt) {
   __shared__ double red[256];
   const uint64_t b0 = (uint64_t)blockIdx.x << bb;
   const uint64_t len = 1ull << bb;
@@ -344,7 +345,10 @@ __global__ __launch_bounds__(256) void k_block_mass(const typename CT<R>::T* __r
   for (uint64_t q = threadIdx.x; q < len; q += blockDim.x) {
     const uint64_t i = b0 + q;
     if (i < dim) {
-      const double re = (double)psi[i].x, im = (double)psi[i].y;
+      typename CT<R>::T v;
+      if (nt) load_nt(v, &psi[i]);
+      else v = psi[i];
+      const double re = (double)v.x, im = (double)v.y;
       s += re * re + im * im;
     }
   }
@@ -381,7 +385,8 @@ __device__ __forceinline__ double units_of(double p, double scale, bool& tie) {
 template <typename R>
 __global__ __launch_bounds__(256) void k_block_exact_sums(const typename CT<R>::T* __restrict__ psi, uint64_t dim,
                                                           int bb, const int32_t* __restrict__ ebase,
-                                                          double* __restrict__ sums, uint32_t* __restrict__ flags) {
+                                                          double* __restrict__ sums, uint32_t* __restrict__ flags,
+                                                          int nt) {
 #pragma clang fp contract(off)
   __shared__ double red[3][256];
   __shared__ uint32_t fred[256];
@@ -401,7 +406,10 @@ __global__ __launch_bounds__(256) void k_block_exact_sums(const typename CT<R>::
   for (uint64_t q = threadIdx.x; q < len; q += blockDim.x) {
     const uint64_t i = b0 + q;
     if (i >= dim) break;
-    const double re = (double)psi[i].x, im = (double)psi[i].y;
+    typename CT<R>::T v;
+    if (nt) load_nt(v, &psi[i]);
+    else v = psi[i];
+    const double re = (double)v.x, im = (double)v.y;
     const double p = re * re + im * im;
     bool t0, t1, t2;
     acc0 += units_of(p, sc[0], t0);
@@ -917,12 +925,17 @@ inline unsigned grid_for(uint64_t work, int block, unsigned cap = 1u << 20) {
 int tile_max_bits(int prec) { return 13; }
 int tile_reg_bits(int K) { return K < 4 ? K : 4; }
 
+// States of at least 512 MB (twice the memory-side cache) are streamed with non-temporal
+// accesses: no later pass finds them in a cache anyway.
+inline int streamed(int prec, uint64_t dim) { return dim * (prec == 0 ? 16u : 8u) >= (512ull << 20) ? 1 : 0; }
+
 hipError_t launch_init_basis(int prec, void* psi, uint64_t dim, uint64_t one_at, hipStream_t s) {
   const unsigned g = grid_for(dim, 256, 1u << 16);
+  const int nt = streamed(prec, dim);
   if (prec == 0)
-    dev::k_init_basis<double><<<g, 256, 0, s>>>((double2*)psi, dim, one_at);
+    dev::k_init_basis<double><<<g, 256, 0, s>>>((double2*)psi, dim, one_at, nt);
   else
-    dev::k_init_basis<float><<<g, 256, 0, s>>>((float2*)psi, dim, one_at);
+    dev::k_init_basis<float><<<g, 256, 0, s>>>((float2*)psi, dim, one_at, nt);
   return hipGetLastError();
 }
 
@@ -1035,20 +1048,24 @@ hipError_t launch_sample_fold(int prec, const void* psi, uint64_t dim, const dou
 
 hipError_t launch_block_mass(int prec, const void* psi, uint64_t dim, int bb, double* mass, hipStream_t s) {
   const uint64_t nb = (dim + (1ull << bb) - 1) >> bb;
+  const int nt = streamed(prec, dim);
   if (prec == 0)
-    dev::k_block_mass<double><<<(unsigned)nb, 256, 0, s>>>((const double2*)psi, dim, bb, mass);
+    dev::k_block_mass<double><<<(unsigned)nb, 256, 0, s>>>((const double2*)psi, dim, bb, mass, nt);
   else
-    dev::k_block_mass<float><<<(unsigned)nb, 256, 0, s>>>((const float2*)psi, dim, bb, mass);
+    dev::k_block_mass<float><<<(unsigned)nb, 256, 0, s>>>((const float2*)psi, dim, bb, mass, nt);
   return hipGetLastError();
 }
 
 hipError_t launch_block_exact_sums(int prec, const void* psi, uint64_t dim, int bb, const int32_t* ebase,
                                    double* sums, uint32_t* flags, hipStream_t s) {
   const uint64_t nb = (dim + (1ull << bb) - 1) >> bb;
+  const int nt = streamed(prec, dim);
   if (prec == 0)
-    dev::k_block_exact_sums<double><<<(unsigned)nb, 256, 0, s>>>((const double2*)psi, dim, bb, ebase, sums, flags);
+    dev::k_block_exact_sums<double>
+        <<<(unsigned)nb, 256, 0, s>>>((const double2*)psi, dim, bb, ebase, sums, flags, nt);
   else
-    dev::k_block_exact_sums<float><<<(unsigned)nb, 256, 0, s>>>((const float2*)psi, dim, bb, ebase, sums, flags);
+    dev::k_block_exact_sums<float>
+        <<<(unsigned)nb, 256, 0, s>>>((const float2*)psi, dim, bb, ebase, sums, flags, nt);
   return hipGetLastError();
 }
 
