@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <fstream>
 #include <cstring>
@@ -238,9 +239,14 @@ static std::vector<Stage> plan_balanced_cached(uint32_t n, const std::vector<Fas
       q.max_ops = (int)cap;
       q.seed = (uint32_t)seed;
       q.tie_slack = (int)slack;
+      if (const char* e = std::getenv("QR_JIT_CACHE_LOG"))
+        if (e[0] == '1') std::fprintf(stderr, "[qr plan] plan choice from the disk cache\n");
       return plan_fused(n, ops, q, B, dual);
     }
   }
+  if (!path.empty())
+    if (const char* e = std::getenv("QR_JIT_CACHE_LOG"))
+      if (e[0] == '1') std::fprintf(stderr, "[qr plan] searching plans\n");
   std::vector<PlanParams> chosen;
   std::vector<std::vector<Stage>> c = plan_fused_candidates(n, ops, pp, B, dual, 1, &chosen);
   if (!path.empty() && !chosen.empty())

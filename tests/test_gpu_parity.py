@@ -393,13 +393,18 @@ st = _capi.State(20)
 st.set_options(fused=1, jit=1)
 st.apply(W.to_program(W.brickwork(20, depth=12, seed=2020)))
 print("digest", hashlib.sha256(st.amplitudes().tobytes()).hexdigest(), flush=True)
+st.close()
+big = _capi.State(25)  # large enough for the plan search, whose choice is cached too
+big.set_options(fused=1, jit=1)
+big.apply(W.to_program(W.brickwork(25, depth=8, seed=2525)))
+print("digest25", hashlib.sha256(big.amplitudes().tobytes()).hexdigest(), flush=True)
 """
 
 
 def test_stage_cache_on_disk(tmp_path):
-    """Compiled stages are kept on disk: a second process with the same cache directory loads
-    every stage of the same circuit from it, compiles none, and gets the same amplitudes bit for
-    bit."""
+    """Compiled stages and plan choices are kept on disk: a second process with the same cache
+    directory loads every stage of the same circuits from it, compiles none, takes the plan
+    search's choice without searching, and gets the same amplitudes bit for bit."""
     import os
     import subprocess
     env = dict(os.environ, QR_JIT_CACHE=str(tmp_path / "jit"), QR_JIT_CACHE_LOG="1")
@@ -410,13 +415,15 @@ def test_stage_cache_on_disk(tmp_path):
                            timeout=240)
         assert r.returncode == 0, r.stderr[-2000:]
         logs = [l for l in r.stderr.splitlines() if l.startswith("[qr jit]")]
-        say("stage cache: %s" % logs)
+        plans = [l for l in r.stderr.splitlines() if l.startswith("[qr plan]")]
+        say("stage cache: %s %s" % (logs, plans))
         digest = [l for l in r.stdout.splitlines() if l.startswith("digest")]
-        runs.append((logs, digest))
+        runs.append((logs, digest, plans))
     first, second = runs
     assert first[0] and " 0 kernels from the disk cache" in first[0][0]
     assert second[0] and all(l.endswith(", 0 to compile") for l in second[0])
-    assert first[1] == second[1] and first[1]
+    assert "[qr plan] searching plans" in first[2] and "[qr plan] plan choice from the disk cache" in second[2]
+    assert first[1] == second[1] and len(first[1]) == 2
 
 
 def test_norm_and_expectation():
