@@ -384,6 +384,12 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
   std::vector<Masks> mk(N);
   std::vector<uint8_t> eligible(N);
   const int k = std::min<int>(pp.tile_bits, (int)n);
+  // Rotations with cos or sin factored out take 4 fused multiply-adds per amplitude pair instead
+  // of 8, but pick their variant with a uniform branch whose register copies and spills cost more
+  // than that in complex128 (30-qubit brickwork 163.8 -> 162.0 ms without them); complex64 keeps
+  // them (95.3 against 99.0 ms). QR_ROT_FORMS=0 or 1 forces either.
+  const char* rot_env = std::getenv("QR_ROT_FORMS");
+  const bool rot = rot_env ? rot_env[0] != '0' : bytes_per_amp == 8;
   std::vector<uint64_t> cmask(N);
   for (int i = 0; i < N; ++i) cmask[i] = ops[i].cmask;
   for (int i = 0; i < N; ++i) {
@@ -773,10 +779,10 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
           const bool free = !dual && t.ns == 0 && t.lane_cmask == 0;  // uncontrolled: scalar is global
           if (x) {
             form = TF_X;
-          } else if (free && op.sub == SUB_RROT) {
+          } else if (rot && free && op.sub == SUB_RROT) {
             form = TF_RROT;
             t.fold = FOLD_RROT;
-          } else if (free && op.sub == SUB_XROT) {
+          } else if (rot && free && op.sub == SUB_XROT) {
             form = TF_XROT;
             t.fold = FOLD_XROT;
           } else if (op.sub == SUB_RX || op.sub == SUB_XROT) {

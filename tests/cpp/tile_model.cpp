@@ -58,8 +58,10 @@ static double rec(cdv l, cdv p) { return l.real() * p.real() + l.imag() * p.imag
 // global bits and tile positions, per-lane / register-slot control split, swizzled LDS).
 // With `lam` (the adjoint sweep) both states go through every op and gradient taps add
 // sum_j Im(conj(lam_j) (G psi)_j) into grads[param].
+// With `cf` (the launch coefficients of stage_coefficients: folded residual matrices, the
+// stage's scalar moved onto one op) ops take their coefficients from there, as the kernels do.
 static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, std::vector<cdv>* lam = nullptr,
-                     std::vector<double>* grads = nullptr) {
+                     std::vector<double>* grads = nullptr, const std::vector<cd>* cf = nullptr) {
   const int K = s.k, NT = K - RB;
   const uint64_t ntiles = 1ull << (n - K);
   uint64_t tm = 0;
@@ -98,7 +100,9 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
       }
     std::vector<std::vector<std::vector<cdv>>*> sets{&reg};
     if (lam) sets.push_back(&lreg);
-    for (const TileOpHost& op : s.ops) {
+    for (size_t oi = 0; oi < s.ops.size(); ++oi) {
+      const TileOpHost& op = s.ops[oi];
+      const cd* coef = cf ? &(*cf)[4 * oi] : op.coef;
       if (op.code == TC_GRAD) {
         if (!lam || !grads) throw Error{QR_ERR_HIP, "model: gradient tap in a single-state stage"};
         const int gen = (int)op.flags;
@@ -141,15 +145,15 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
               if (((uint32_t)m0 & op.ns) != op.ns) continue;
               const int m1 = m0 | (1 << J);
               const cdv x = reg[t][m0], y = reg[t][m1];
-              reg[t][m0] = op.coef[0] * x + op.coef[1] * y;
-              reg[t][m1] = op.coef[2] * x + op.coef[3] * y;
+              reg[t][m0] = coef[0] * x + coef[1] * y;
+              reg[t][m1] = coef[2] * x + coef[3] * y;
             }
           } else {
             for (int m = 0; m < R; ++m) {
               if (((uint32_t)m & op.ns) != op.ns) continue;
               const bool bit = (op.b != 0xFFu) ? ((m >> op.b) & 1) : ((g0 >> op.a) & 1);
               if ((op.flags & 1) && !bit) continue;
-              reg[t][m] *= bit ? op.coef[1] : op.coef[0];
+              reg[t][m] *= bit ? coef[1] : coef[0];
             }
           }
         }
@@ -157,7 +161,7 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
       if (op.code == TC_SCALE) {
         for (auto* rs : sets)
           for (auto& th : *rs)
-            for (auto& v : th) v *= op.coef[0];
+            for (auto& v : th) v *= coef[0];
       }
       if (op.code == TC_REMAP) {
         const uint16_t* P = &s.swz[(size_t)op.swz * 2 * K];
@@ -261,9 +265,14 @@ int model_run(uint32_t n, double* state, const qr_program* prog, int tile_bits, 
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);
+    std::vector<cd> cf;
     for (const Stage& s : stages) {
-      if (s.tile) run_tile(n, psi, s, pp.reg_bits);
-      else apply_standalone(n, psi, ops[s.standalone]);
+      if (s.tile) {
+        stage_coefficients(s, ops, cf);
+        run_tile(n, psi, s, pp.reg_bits, nullptr, nullptr, &cf);
+      } else {
+        apply_standalone(n, psi, ops[s.standalone]);
+      }
     }
     std::memcpy(state, psi.data(), psi.size() * 16);
     if (sweeps_out) *sweeps_out = stages.size();
@@ -292,9 +301,14 @@ int model_run_opts(uint32_t n, double* state, const qr_program* prog, int tile_b
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);
+    std::vector<cd> cf;
     for (const Stage& s : stages) {
-      if (s.tile) run_tile(n, psi, s, pp.reg_bits);
-      else apply_standalone(n, psi, ops[s.standalone]);
+      if (s.tile) {
+        stage_coefficients(s, ops, cf);
+        run_tile(n, psi, s, pp.reg_bits, nullptr, nullptr, &cf);
+      } else {
+        apply_standalone(n, psi, ops[s.standalone]);
+      }
     }
     std::memcpy(state, psi.data(), psi.size() * 16);
     if (sweeps_out) *sweeps_out = stages.size();
@@ -318,9 +332,14 @@ int model_run_capped(uint32_t n, double* state, const qr_program* prog, int tile
     auto stages = plan_fused(n, ops, pp, 16, false);
     std::vector<cdv> psi(1ull << n);
     std::memcpy(psi.data(), state, psi.size() * 16);
+    std::vector<cd> cf;
     for (const Stage& s : stages) {
-      if (s.tile) run_tile(n, psi, s, pp.reg_bits);
-      else apply_standalone(n, psi, ops[s.standalone]);
+      if (s.tile) {
+        stage_coefficients(s, ops, cf);
+        run_tile(n, psi, s, pp.reg_bits, nullptr, nullptr, &cf);
+      } else {
+        apply_standalone(n, psi, ops[s.standalone]);
+      }
     }
     std::memcpy(state, psi.data(), psi.size() * 16);
     if (sweeps_out) *sweeps_out = stages.size();

@@ -4,6 +4,7 @@ tests/cpp/tile_model.cpp runs the engine's own lowering and planner and replays 
 stage with the kernel's per-thread index arithmetic, so planner mistakes (a dropped gate, a
 reordered dependency, a wrong slot) fail here without a GPU."""
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -72,6 +73,30 @@ def test_model_brickwork_and_qft():
     p = W.to_program(gates)
     got, _ = run_model(10, p, 8)
     assert np.max(np.abs(got - H.oracle_simulate(10, p))) < 1e-12
+
+
+@pytest.mark.parametrize("rot", ["0", "1"])
+def test_model_rotation_forms(rot, monkeypatch):
+    """Rx and Ry as rotations with cos or sin divided out (4 fused multiply-adds per pair, one
+    variant per sign of |cos| - |sin|; complex64's default) and as plain 2x2 forms (complex128's
+    default), run through the launch coefficients, against the oracle. The angles take both
+    variants, the crossover at pi/2 and the zeros of cos and sin."""
+    monkeypatch.setenv("QR_ROT_FORMS", rot)
+    n = 9
+    angles = [0.0, 1e-12, 0.3, math.pi / 4, math.pi / 2, math.pi / 2 + 1e-9, 2.0, math.pi, 4.2,
+              3 * math.pi / 2, 5.9, 2 * math.pi]
+    gates = []
+    for layer in range(6):
+        for w in range(n):
+            kind = ("Rx", "Ry", "H")[(w + layer) % 3]
+            gates.append((kind, w, (), angles[(5 * w + layer) % len(angles)] if kind != "H" else 0.0, 0))
+        for w in range(layer % 2, n - 1, 2):
+            gates.append(("CNot", w + 1, (w,), 0.0, 0))
+    p = H.to_program(gates)
+    want = H.oracle_simulate(n, p)
+    for tb in (3, 6, 9):
+        got, _ = run_model(n, p, tb)
+        assert np.max(np.abs(got - want)) < 1e-12, tb
 
 
 def test_model_whole_state_stage_limit():
