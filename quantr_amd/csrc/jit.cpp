@@ -604,10 +604,20 @@ bool write_file_atomic(const std::string& path, const char* data, size_t size) {
 std::map<std::string, hipModule_t> g_disk_modules;  // object file -> module (under g_mu)
 
 // The kernel for `kernel` from the disk cache, if there. Caller holds no lock.
+std::string content_hash(const std::vector<char>& code) {
+  Hash128 h;
+  h.add(std::string(code.begin(), code.end()));
+  return h.hex();
+}
+
+// An index file names the code object, the kernel's symbol in it and the object's content
+// hash; an object whose bytes do not hash to that (a damaged or foreign file) is never loaded.
 bool load_cached(const std::string& dir, const std::string& kernel, JitKernel& out) {
   std::ifstream f(dir + "/" + kernel_key(kernel) + ".k");
-  std::string obj, name;
-  if (!f || !std::getline(f, obj) || !std::getline(f, name) || obj.empty() || name.empty()) return false;
+  std::string obj, name, sum;
+  if (!f || !std::getline(f, obj) || !std::getline(f, name) || !std::getline(f, sum) || obj.empty() ||
+      name.empty() || sum.size() != 32)
+    return false;
   if (obj.find('/') != std::string::npos) return false;
   std::lock_guard<std::mutex> lk(g_mu);
   hipModule_t mod = nullptr;
@@ -616,7 +626,8 @@ bool load_cached(const std::string& dir, const std::string& kernel, JitKernel& o
     mod = it->second;
   } else {
     std::vector<char> code;
-    if (!read_file(dir + "/" + obj, code) || hipModuleLoadData(&mod, code.data()) != hipSuccess) return false;
+    if (!read_file(dir + "/" + obj, code) || content_hash(code) != sum) return false;
+    if (hipModuleLoadData(&mod, code.data()) != hipSuccess) return false;
     g_disk_modules[obj] = mod;
   }
   hipFunction_t fn = nullptr;
@@ -631,11 +642,12 @@ void store_group(const std::string& dir, const std::vector<std::string>& todo, c
   Hash128 gh;
   for (size_t j = 0; j < G.items.size(); ++j) gh.add(todo[G.items[j]]);
   const std::string obj = "g" + gh.hex() + ".co";
-  if (access((dir + "/" + obj).c_str(), R_OK) != 0 && !write_file_atomic(dir + "/" + obj, code.data(), code.size()))
-    return;
+  // written even where the file exists, so that a damaged copy is replaced
+  if (!write_file_atomic(dir + "/" + obj, code.data(), code.size())) return;
+  const std::string sum = content_hash(code);
   for (size_t j = 0; j < G.items.size(); ++j) {
     const std::string& k = todo[G.items[j]];
-    const std::string idx = obj + "\n" + entry_name(k) + "_" + std::to_string(j) + "\n";
+    const std::string idx = obj + "\n" + entry_name(k) + "_" + std::to_string(j) + "\n" + sum + "\n";
     (void)write_file_atomic(dir + "/" + kernel_key(k) + ".k", idx.data(), idx.size());
   }
 }
@@ -713,7 +725,7 @@ void jit_prepare(const std::vector<std::string>& sources) {
     if (!dir.empty()) {
       const std::string key = kernel_key(todo[i]), obj = "s" + key + ".co";
       if (write_file_atomic(dir + "/" + obj, code.data(), code.size())) {
-        const std::string idx = obj + "\n" + entry_name(todo[i]) + "\n";
+        const std::string idx = obj + "\n" + entry_name(todo[i]) + "\n" + content_hash(code) + "\n";
         (void)write_file_atomic(dir + "/" + key + ".k", idx.data(), idx.size());
       }
     }

@@ -404,12 +404,20 @@ print("digest25", hashlib.sha256(big.amplitudes().tobytes()).hexdigest(), flush=
 def test_stage_cache_on_disk(tmp_path):
     """Compiled stages and plan choices are kept on disk: a second process with the same cache
     directory loads every stage of the same circuits from it, compiles none, takes the plan
-    search's choice without searching, and gets the same amplitudes bit for bit."""
+    search's choice without searching, and gets the same amplitudes bit for bit. A damaged code
+    object (cut to half its size) fails its content hash and is never loaded: a third process
+    compiles its kernels again, rewrites it, and gets the same amplitudes."""
+    import glob
     import os
     import subprocess
     env = dict(os.environ, QR_JIT_CACHE=str(tmp_path / "jit"), QR_JIT_CACHE_LOG="1")
     runs = []
-    for k in range(2):
+    for k in range(3):
+        if k == 2:
+            objs = sorted(glob.glob(str(tmp_path / "jit" / "*" / "g*.co")) + glob.glob(str(tmp_path / "jit" / "g*.co")))
+            assert objs
+            with open(objs[0], "r+b") as f:
+                f.truncate(os.path.getsize(objs[0]) // 2)
         say("stage cache: process %d" % k)
         r = subprocess.run([sys.executable, "-c", _CACHE_SCRIPT, H.ROOT], env=env, capture_output=True, text=True,
                            timeout=240)
@@ -419,11 +427,13 @@ def test_stage_cache_on_disk(tmp_path):
         say("stage cache: %s %s" % (logs, plans))
         digest = [l for l in r.stdout.splitlines() if l.startswith("digest")]
         runs.append((logs, digest, plans))
-    first, second = runs
+    first, second, third = runs
     assert first[0] and " 0 kernels from the disk cache" in first[0][0]
     assert second[0] and all(l.endswith(", 0 to compile") for l in second[0])
     assert "[qr plan] searching plans" in first[2] and "[qr plan] plan choice from the disk cache" in second[2]
     assert first[1] == second[1] and len(first[1]) == 2
+    assert any(not l.endswith(", 0 to compile") for l in third[0])
+    assert third[1] == first[1]
 
 
 def test_norm_and_expectation():
