@@ -60,8 +60,8 @@ enum TileForm : int {
   TF_RX = 4,      // real diagonal, imaginary off-diagonal (Rx)
   TF_HAD = 5,     // [[1,1],[1,-1]] (H with its scalar factored out)
   TF_NEGSWAP = 6, // [[0,-1],[1,0]] (Y, Y90, MY90 with their scalar factored out)
-  TF_RROT = 7,    // [[1,-t],[t,1]] or [[u,-1],[1,u]] (Ry with cos or sin factored out)
-  TF_XROT = 8,    // [[1,-it],[-it,1]] or [[iu,1],[1,iu]] (Rx with cos or -i sin factored out)
+  TF_RROT = 7,    // real rotation as three shears, al = c[0], be = c[1] (Ry; magnitude folded out)
+  TF_XROT = 8,    // the same for [[c,-is],[-is,c]], the rotation of (x, i y) (Rx)
 };
 
 // Exact value classes of an op, used to factor scalars out of uncontrolled gates.
@@ -150,8 +150,8 @@ struct TileOpHost {
 // How an op takes part in a stage's scalar folding: its factor is divided out of its
 // coefficients (FOLD_M00: m00, FOLD_M10: m10, FOLD_D0: d0), and the product of the stage's
 // factors is multiplied into one general-form op (FOLD_ABSORB) or applied by TC_SCALE.
-// FOLD_RROT / FOLD_XROT divide out the larger of the two magnitudes (cos or sin), leaving
-// a residual with an exact unit entry that the kernel tests to pick its variant.
+// FOLD_RROT / FOLD_XROT divide out the magnitude and a sign, leaving a rotation by at most
+// pi/2 that the kernel applies as three shears.
 enum FoldMode : int {
   FOLD_NONE = 0, FOLD_M00 = 1, FOLD_M10 = 2, FOLD_D0 = 3, FOLD_ABSORB = 4, FOLD_SCALE = 5, FOLD_RROT = 6,
   FOLD_XROT = 7

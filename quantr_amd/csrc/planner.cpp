@@ -384,12 +384,10 @@ std::vector<Stage> plan_fused(uint32_t n, const std::vector<FastOp>& ops, const 
   std::vector<Masks> mk(N);
   std::vector<uint8_t> eligible(N);
   const int k = std::min<int>(pp.tile_bits, (int)n);
-  // Rotations with cos or sin factored out take 4 fused multiply-adds per amplitude pair instead
-  // of 8, but pick their variant with a uniform branch whose register copies and spills cost more
-  // than that in complex128 (30-qubit brickwork 163.8 -> 162.0 ms without them); complex64 keeps
-  // them (95.3 against 99.0 ms). QR_ROT_FORMS=0 or 1 forces either.
+  // Uncontrolled Ry and Rx run as three shears (6 fused multiply-adds per amplitude pair, no
+  // branch) instead of plain 2x2 forms (8); QR_ROT_FORMS=0 turns this off.
   const char* rot_env = std::getenv("QR_ROT_FORMS");
-  const bool rot = rot_env ? rot_env[0] != '0' : bytes_per_amp == 8;
+  const bool rot = rot_env ? rot_env[0] != '0' : true;
   std::vector<uint64_t> cmask(N);
   for (int i = 0; i < N; ++i) cmask[i] = ops[i].cmask;
   for (int i = 0; i < N; ++i) {
@@ -960,28 +958,21 @@ void stage_coefficients(const Stage& s, const std::vector<FastOp>& ops, std::vec
       o[1] = m[1] / f;
       o[0] = cd(1, 0);
       total *= f;
-    } else if (t.fold == FOLD_RROT) {  // [[a,-b],[b,a]]
-      const double a = m[0].real(), b = m[2].real();
-      if (std::fabs(a) >= std::fabs(b)) {
-        const double tq = b / a;
-        o[0] = cd(1, 0); o[1] = cd(-tq, 0); o[2] = cd(tq, 0); o[3] = cd(1, 0);
-        total *= a;
-      } else {
-        const double u = a / b;
-        o[0] = cd(u, 0); o[1] = cd(-1, 0); o[2] = cd(1, 0); o[3] = cd(u, 0);
-        total *= b;
+    } else if (t.fold == FOLD_RROT || t.fold == FOLD_XROT) {
+      // [[a,-b],[b,a]] (RROT), or [[a,-ib],[-ib,a]] (XROT: the same rotation of (x, i y)),
+      // = r * rotation(phi). phi is reduced to [-pi/2, pi/2] (a >= 0, the sign folded out),
+      // then al = -tan(phi/2) = -b / (r + a) and be = sin(phi) = b / r, both in [-1, 1].
+      double a = m[0].real(), b = t.fold == FOLD_RROT ? m[2].real() : -m[1].imag();
+      double sgn = 1.0;
+      if (a < 0) {
+        a = -a;
+        b = -b;
+        sgn = -1.0;
       }
-    } else if (t.fold == FOLD_XROT) {  // [[a,-ib],[-ib,a]]
-      const double a = m[0].real(), b = -m[1].imag();
-      if (std::fabs(a) >= std::fabs(b)) {
-        const double tq = b / a;
-        o[0] = cd(1, 0); o[1] = cd(0, -tq); o[2] = cd(0, -tq); o[3] = cd(1, 0);
-        total *= a;
-      } else {
-        const double u = a / b;
-        o[0] = cd(0, u); o[1] = cd(1, 0); o[2] = cd(1, 0); o[3] = cd(0, u);
-        total *= cd(0, -b);
-      }
+      const double r = std::hypot(a, b);
+      const double al = r > 0 ? -b / (r + a) : 0.0, be = r > 0 ? b / r : 0.0;
+      o[0] = cd(al, 0); o[1] = cd(be, 0); o[2] = cd(0, 0); o[3] = cd(0, 0);
+      total *= sgn * r;
     }
   }
   if (absorber >= 0)

@@ -49,41 +49,40 @@ __device__ __forceinline__ void cscale(typename CT<R>::T& x, R fr, R fi) {
   }
 }
 
-// Residual rotations (scalars factored out): 4 fused multiply-adds per amplitude pair.
-// TAN: [[1,-t],[t,1]]; else [[u,-1],[1,u]].
-template <typename R, int RB, int J, bool TAN, bool HAS_NS>
-__device__ __forceinline__ void rot_real(Regs<R, RB>& a, R t, uint32_t ns) {
+// Rotations as three shears: x += al*y, y += be*x, x += al*y with al = -tan(phi/2) and
+// be = sin(phi) give [[cos phi, -sin phi], [sin phi, cos phi]]. The host reduces phi to
+// [-pi/2, pi/2] (so |al|, |be| <= 1) and folds the sign and the magnitude into the stage's
+// scalar: 6 fused multiply-adds per amplitude pair and no branch on the angle.
+template <typename R, int RB, int J, bool HAS_NS>
+__device__ __forceinline__ void rot_real(Regs<R, RB>& a, R al, R be, uint32_t ns) {
 #pragma unroll
   for (int m0 = 0; m0 < (1 << RB); ++m0) {
     if (m0 & (1 << J)) continue;
     if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
     const int m1 = m0 | (1 << J);
-    const R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
-    if (TAN) {
-      a.v[m0].x = fma(-t, yr, xr); a.v[m0].y = fma(-t, yi, xi);
-      a.v[m1].x = fma(t, xr, yr);  a.v[m1].y = fma(t, xi, yi);
-    } else {
-      a.v[m0].x = fma(t, xr, -yr); a.v[m0].y = fma(t, xi, -yi);
-      a.v[m1].x = fma(t, yr, xr);  a.v[m1].y = fma(t, yi, xi);
-    }
+    R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
+    xr = fma(al, yr, xr); xi = fma(al, yi, xi);
+    yr = fma(be, xr, yr); yi = fma(be, xi, yi);
+    xr = fma(al, yr, xr); xi = fma(al, yi, xi);
+    a.v[m0].x = xr; a.v[m0].y = xi;
+    a.v[m1].x = yr; a.v[m1].y = yi;
   }
 }
-// TAN: [[1,-it],[-it,1]]; else [[iu,1],[1,iu]].
-template <typename R, int RB, int J, bool TAN, bool HAS_NS>
-__device__ __forceinline__ void rot_x(Regs<R, RB>& a, R t, uint32_t ns) {
+// [[cos, -i sin], [-i sin, cos]] is the real rotation of the pair (x, i y): the same shears
+// with y's real and imaginary parts exchanged.
+template <typename R, int RB, int J, bool HAS_NS>
+__device__ __forceinline__ void rot_x(Regs<R, RB>& a, R al, R be, uint32_t ns) {
 #pragma unroll
   for (int m0 = 0; m0 < (1 << RB); ++m0) {
     if (m0 & (1 << J)) continue;
     if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
     const int m1 = m0 | (1 << J);
-    const R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
-    if (TAN) {  // x - i t y, y - i t x
-      a.v[m0].x = fma(t, yi, xr);  a.v[m0].y = fma(-t, yr, xi);
-      a.v[m1].x = fma(t, xi, yr);  a.v[m1].y = fma(-t, xr, yi);
-    } else {    // i u x + y, x + i u y
-      a.v[m0].x = fma(-t, xi, yr); a.v[m0].y = fma(t, xr, yi);
-      a.v[m1].x = fma(-t, yi, xr); a.v[m1].y = fma(t, yr, xi);
-    }
+    R xr = a.v[m0].x, xi = a.v[m0].y, yr = a.v[m1].x, yi = a.v[m1].y;
+    xr = fma(-al, yi, xr); xi = fma(al, yr, xi);
+    yi = fma(-be, xr, yi); yr = fma(be, xi, yr);
+    xr = fma(-al, yi, xr); xi = fma(al, yr, xi);
+    a.v[m0].x = xr; a.v[m0].y = xi;
+    a.v[m1].x = yr; a.v[m1].y = yi;
   }
 }
 
@@ -92,17 +91,9 @@ template <typename R, int RB, int J, int FORM, bool HAS_NS>
 __device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const Coef4 u, uint32_t ns) {
   const R u00r = (R)u.c[0].x, u00i = (R)u.c[0].y, u01r = (R)u.c[1].x, u01i = (R)u.c[1].y;
   const R u10r = (R)u.c[2].x, u10i = (R)u.c[2].y, u11r = (R)u.c[3].x, u11i = (R)u.c[3].y;
-  if (FORM == 7 || FORM == 8) {  // rotations with a unit entry: a uniform choice of variant
-    // compared as integers: the coefficients sit in scalar registers, and an integer test
-    // keeps the choice a uniform branch (a floating-point test would run per lane)
-    const bool tanv = __double_as_longlong(u.c[0].x) == 0x3FF0000000000000LL && __double_as_longlong(u.c[0].y) == 0;
-    if (FORM == 7) {
-      if (tanv) rot_real<R, RB, J, true, HAS_NS>(a, u10r, ns);
-      else rot_real<R, RB, J, false, HAS_NS>(a, u00r, ns);
-    } else {
-      if (tanv) rot_x<R, RB, J, true, HAS_NS>(a, -u01i, ns);
-      else rot_x<R, RB, J, false, HAS_NS>(a, u00i, ns);
-    }
+  if (FORM == 7 || FORM == 8) {  // rotations as shears: al in c[0], be in c[1]
+    if (FORM == 7) rot_real<R, RB, J, HAS_NS>(a, u00r, u01r, ns);
+    else rot_x<R, RB, J, HAS_NS>(a, u00r, u01r, ns);
     return;
   }
 #pragma unroll

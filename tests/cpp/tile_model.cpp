@@ -140,11 +140,25 @@ static void run_tile(uint32_t n, std::vector<cdv>& psi, const Stage& s, int RB, 
           if (!lane_ok) continue;
           if (op.code == TC_U1) {
             const int J = (int)op.a;
+            // folded rotations hold the kernel's shear factors: x += al y, y += be x, x += al y
+            // on (x, y) (TF_RROT) or on (x, i y) (TF_XROT)
+            const bool shear = cf && (op.b == TF_RROT || op.b == TF_XROT);
+            const cdv w = op.b == TF_XROT ? cdv(0, 1) : cdv(1, 0);
             for (int m0 = 0; m0 < R; ++m0) {
               if (m0 & (1 << J)) continue;
               if (((uint32_t)m0 & op.ns) != op.ns) continue;
               const int m1 = m0 | (1 << J);
               const cdv x = reg[t][m0], y = reg[t][m1];
+              if (shear) {
+                const double al = coef[0].real(), be = coef[1].real();
+                cdv xs = x, ys = w * y;
+                xs += al * ys;
+                ys += be * xs;
+                xs += al * ys;
+                reg[t][m0] = xs;
+                reg[t][m1] = ys / w;
+                continue;
+              }
               reg[t][m0] = coef[0] * x + coef[1] * y;
               reg[t][m1] = coef[2] * x + coef[3] * y;
             }
