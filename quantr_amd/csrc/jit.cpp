@@ -286,8 +286,8 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             if (s.maps[(size_t)nmi * K + sl] == h) nsl = sl;
           }
           const uint32_t hm = (1u << (K - 1)) - 1;
-          o << "  { const uint32_t w = (" << lds_thread(P) << ") & " << hm << "u;\n";
-          o << "    const uint32_t q = (" << lds_thread(P + K) << ") & " << hm << "u;\n";
+          o << "  { const uint32_t w = ((" << lds_thread(P) << ") & " << hm << "u) * " << amp_bytes << "u;\n";
+          o << "    const uint32_t q = ((" << lds_thread(P + K) << ") & " << hm << "u) * " << amp_bytes << "u;\n";
           auto in_round = [&](int slot, int m, int rho) {  // register m of this half (register slot)
             return ((m >> (slot - NT)) & 1) == rho;
           };
@@ -300,15 +300,15 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             if (osl < NT) o << "    if (((tid >> " << osl << ") & 1u) == " << rho << "u) {\n";
             for (int m = 0; m < NR; ++m)
               if (osl < NT || in_round(osl, m, rho))
-                o << "    lds[w ^ " << (lds_reg(P, m) & hm) << "u] = r.v[" << m << "];\n";
+                o << "    qr::dev::lds_at(lds, w ^ " << (lds_reg(P, m) & hm) * amp_bytes << "u) = r.v[" << m << "];\n";
             if (osl < NT) o << "    }\n";
             o << "    __syncthreads();\n";
             if (nsl < NT) o << "    if (((tid >> " << nsl << ") & 1u) == " << rho << "u) {\n";
             for (int m = 0; m < NR; ++m) {
               if (!(nsl < NT || in_round(nsl, m, rho))) continue;
               const bool t = rho == 0 && std::find(tmp.begin(), tmp.end(), m) != tmp.end();
-              o << "    " << (t ? "t" + std::to_string(m) : "r.v[" + std::to_string(m) + "]") << " = lds[q ^ "
-                << (lds_reg(P + K, m) & hm) << "u];\n";
+              o << "    " << (t ? "t" + std::to_string(m) : "r.v[" + std::to_string(m) + "]")
+                << " = qr::dev::lds_at(lds, q ^ " << (lds_reg(P + K, m) & hm) * amp_bytes << "u);\n";
             }
             if (nsl < NT) o << "    }\n";
           }
@@ -321,25 +321,27 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
           o << "    g0 = " << thread_part(mi) << "; }\n";
           break;
         }
-        o << "  { const uint32_t w = " << lds_thread(P) << ";\n";
-        o << "    const uint32_t q = " << lds_thread(P + K) << ";\n";
+        o << "  { const uint32_t w = (" << lds_thread(P) << ") * " << amp_bytes << "u;\n";
+        o << "    const uint32_t q = (" << lds_thread(P + K) << ") * " << amp_bytes << "u;\n";
         if (lds2 || !dual) {
           o << "    __syncthreads();\n";
           for (size_t v = 0; v < sets.size(); ++v)
             for (int m = 0; m < NR; ++m)
-              o << "    lds[" << v * (1 << K) << "u + (w ^ " << lds_reg(P, m) << "u)] = " << sets[v] << ".v[" << m
-                << "];\n";
+              o << "    qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes << "u + (w ^ " << lds_reg(P, m) * amp_bytes
+                << "u)) = " << sets[v] << ".v[" << m << "];\n";
           o << "    __syncthreads();\n";
           for (size_t v = 0; v < sets.size(); ++v)
             for (int m = 0; m < NR; ++m)
-              o << "    " << sets[v] << ".v[" << m << "] = lds[" << v * (1 << K) << "u + (q ^ " << lds_reg(P + K, m)
-                << "u)];\n";
+              o << "    " << sets[v] << ".v[" << m << "] = qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes
+                << "u + (q ^ " << lds_reg(P + K, m) * amp_bytes << "u));\n";
         } else {
           for (const auto& v : sets) {
             o << "    __syncthreads();\n";
-            for (int m = 0; m < NR; ++m) o << "    lds[w ^ " << lds_reg(P, m) << "u] = " << v << ".v[" << m << "];\n";
+            for (int m = 0; m < NR; ++m)
+              o << "    qr::dev::lds_at(lds, w ^ " << lds_reg(P, m) * amp_bytes << "u) = " << v << ".v[" << m << "];\n";
             o << "    __syncthreads();\n";
-            for (int m = 0; m < NR; ++m) o << "    " << v << ".v[" << m << "] = lds[q ^ " << lds_reg(P + K, m) << "u];\n";
+            for (int m = 0; m < NR; ++m)
+              o << "    " << v << ".v[" << m << "] = qr::dev::lds_at(lds, q ^ " << lds_reg(P + K, m) * amp_bytes << "u);\n";
           }
         }
         mi = (int)op.a;
