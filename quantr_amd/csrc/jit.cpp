@@ -277,6 +277,29 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             if ((m >> j) & 1) x ^= Q[NT + j];
           return x;
         };
+        // Byte offset of register m's slot: (thread part) ^ (register part). The register part's
+        // bits that no thread pattern can set (H) add, and fold into the access's immediate
+        // offset; its other bits (E, from the swizzle) are XORed into one base per distinct E,
+        // declared here, so an exchange takes a few XORs rather than one per access.
+        auto lds_bases = [&](const char* name, const uint16_t* Q, uint32_t mask) {
+          uint32_t T = 0;
+          for (int sl = 0; sl < NT; ++sl) T |= Q[sl] & mask;
+          std::vector<std::string> ex(NR);
+          std::vector<uint32_t> seen;
+          for (int m = 0; m < NR; ++m) {
+            const uint32_t v = lds_reg(Q, m) & mask, E = v & T, H = v & ~T;
+            std::string b = name;
+            if (E) {
+              b += "_" + std::to_string(E);
+              if (std::find(seen.begin(), seen.end(), E) == seen.end()) {
+                seen.push_back(E);
+                o << "    const uint32_t " << b << " = " << name << " ^ " << E * amp_bytes << "u;\n";
+              }
+            }
+            ex[m] = "(" + b + " + " + std::to_string(H * amp_bytes) + "u)";
+          }
+          return ex;
+        };
         if (split && (op.flags & 1)) {
           // two rounds through half the LDS, split by tile position op.b
           const int h = (int)op.b, nmi = (int)op.a;
@@ -288,6 +311,7 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
           const uint32_t hm = (1u << (K - 1)) - 1;
           o << "  { const uint32_t w = ((" << lds_thread(P) << ") & " << hm << "u) * " << amp_bytes << "u;\n";
           o << "    const uint32_t q = ((" << lds_thread(P + K) << ") & " << hm << "u) * " << amp_bytes << "u;\n";
+          const std::vector<std::string> wa = lds_bases("w", P, hm), qa = lds_bases("q", P + K, hm);
           auto in_round = [&](int slot, int m, int rho) {  // register m of this half (register slot)
             return ((m >> (slot - NT)) & 1) == rho;
           };
@@ -300,7 +324,7 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             if (osl < NT) o << "    if (((tid >> " << osl << ") & 1u) == " << rho << "u) {\n";
             for (int m = 0; m < NR; ++m)
               if (osl < NT || in_round(osl, m, rho))
-                o << "    qr::dev::lds_at(lds, w ^ " << (lds_reg(P, m) & hm) * amp_bytes << "u) = r.v[" << m << "];\n";
+                o << "    qr::dev::lds_at(lds, " << wa[m] << ") = r.v[" << m << "];\n";
             if (osl < NT) o << "    }\n";
             o << "    __syncthreads();\n";
             if (nsl < NT) o << "    if (((tid >> " << nsl << ") & 1u) == " << rho << "u) {\n";
@@ -308,7 +332,7 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
               if (!(nsl < NT || in_round(nsl, m, rho))) continue;
               const bool t = rho == 0 && std::find(tmp.begin(), tmp.end(), m) != tmp.end();
               o << "    " << (t ? "t" + std::to_string(m) : "r.v[" + std::to_string(m) + "]")
-                << " = qr::dev::lds_at(lds, q ^ " << (lds_reg(P + K, m) & hm) * amp_bytes << "u);\n";
+                << " = qr::dev::lds_at(lds, " << qa[m] << ");\n";
             }
             if (nsl < NT) o << "    }\n";
           }
@@ -323,25 +347,26 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
         }
         o << "  { const uint32_t w = (" << lds_thread(P) << ") * " << amp_bytes << "u;\n";
         o << "    const uint32_t q = (" << lds_thread(P + K) << ") * " << amp_bytes << "u;\n";
+        const std::vector<std::string> wa = lds_bases("w", P, 0xFFFFFFFFu), qa = lds_bases("q", P + K, 0xFFFFFFFFu);
         if (lds2 || !dual) {
           o << "    __syncthreads();\n";
           for (size_t v = 0; v < sets.size(); ++v)
             for (int m = 0; m < NR; ++m)
-              o << "    qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes << "u + (w ^ " << lds_reg(P, m) * amp_bytes
-                << "u)) = " << sets[v] << ".v[" << m << "];\n";
+              o << "    qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes << "u + " << wa[m] << ") = " << sets[v]
+                << ".v[" << m << "];\n";
           o << "    __syncthreads();\n";
           for (size_t v = 0; v < sets.size(); ++v)
             for (int m = 0; m < NR; ++m)
-              o << "    " << sets[v] << ".v[" << m << "] = qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes
-                << "u + (q ^ " << lds_reg(P + K, m) * amp_bytes << "u));\n";
+              o << "    " << sets[v] << ".v[" << m << "] = qr::dev::lds_at(lds, " << v * (1 << K) * amp_bytes << "u + "
+                << qa[m] << ");\n";
         } else {
           for (const auto& v : sets) {
             o << "    __syncthreads();\n";
             for (int m = 0; m < NR; ++m)
-              o << "    qr::dev::lds_at(lds, w ^ " << lds_reg(P, m) * amp_bytes << "u) = " << v << ".v[" << m << "];\n";
+              o << "    qr::dev::lds_at(lds, " << wa[m] << ") = " << v << ".v[" << m << "];\n";
             o << "    __syncthreads();\n";
             for (int m = 0; m < NR; ++m)
-              o << "    " << v << ".v[" << m << "] = qr::dev::lds_at(lds, q ^ " << lds_reg(P + K, m) * amp_bytes << "u);\n";
+              o << "    " << v << ".v[" << m << "] = qr::dev::lds_at(lds, " << qa[m] << ");\n";
           }
         }
         mi = (int)op.a;
