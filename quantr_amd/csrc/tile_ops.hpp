@@ -29,9 +29,8 @@ struct Coef4 {
   double2 c[4];
 };
 
-// The LDS element at byte offset `off`: remaps keep per-thread byte offsets and XOR each
-// access's constant into them (one integer instruction, none where the constant's bits are
-// known to be clear in the offset and fold into the access's immediate offset).
+// The LDS element at byte offset `off`. Remaps address LDS by per-thread byte offsets plus
+// constants that fold into the access's immediate offset.
 template <typename T>
 __device__ __forceinline__ T& lds_at(T* base, uint32_t off) {
   return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + off);
