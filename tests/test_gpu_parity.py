@@ -186,6 +186,30 @@ def test_custom_diagonal_zero_entries_gpu(diag, controls):
             assert np.max(np.abs(got - want)) < TOL_FUSED, (diag, controls, tile_bits, jit)
 
 
+@pytest.mark.parametrize("prec,tol", [(_capi.QR_C128, TOL_FUSED), (_capi.QR_C64, TOL_C64)])
+def test_rotation_angles_gpu(prec, tol):
+    """Rx and Ry run as three shears after the host folds out their magnitude and sign: the
+    angles where that fold changes sign or degenerates (0, pi/2, pi, 3pi/2, 2pi and their
+    neighbours), in compiled and interpreted stages, against the oracle."""
+    import math
+    angles = [0.0, 1e-12, 0.3, math.pi / 4, math.pi / 2, math.pi / 2 + 1e-9, 2.0, math.pi,
+              math.pi + 1e-9, 4.2, 3 * math.pi / 2, 5.9, 2 * math.pi - 1e-9, 2 * math.pi]
+    for n in (9, 14):
+        gates = []
+        for layer in range(6):
+            for w in range(n):
+                kind = ("Rx", "Ry", "H")[(w + layer) % 3]
+                gates.append((kind, w, (), angles[(5 * w + layer) % len(angles)] if kind != "H" else 0.0, 0))
+            for w in range(layer % 2, n - 1, 2):
+                gates.append(("CNot", w + 1, (w,), 0.0, 0))
+        p = H.to_program(gates)
+        want = H.oracle_simulate(n, p)
+        for jit in (1, 0):
+            say("rotation angles n", n, "prec", prec, "jit", jit)
+            got = run_gpu(n, p, fused=1, precision=prec, jit=jit)
+            assert np.max(np.abs(got - want)) < tol, (n, jit)
+
+
 def test_complex64_random():
     rng = np.random.default_rng(404)
     for trial in range(10):
