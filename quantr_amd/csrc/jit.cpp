@@ -198,10 +198,10 @@ std::string stage_source(const Stage& s, int prec, int RB, bool persistent, bool
             o << " qr::dev::u1_slot<R, " << RB << ", " << op.a << ", " << op.b << ", " << has_ns << ">(" << v
               << ", u, " << ns << ");";
           o << " }\n";
-        } else if (op.b == TF_X) {  // controlled X: per-lane select, no arithmetic
+        } else if (op.b == TF_X) {  // controlled X: a per-lane exchange, no arithmetic
           o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";";
           for (const auto& v : sets)
-            o << " qr::dev::u1_xsel<R, " << RB << ", " << op.a << ", " << has_ns << ">(" << v << ", ok, " << ns << ");";
+            o << " qr::dev::u1_xswap<R, " << RB << ", " << op.a << ", " << has_ns << ">(" << v << ", ok, " << ns << ");";
           o << " }\n";
         } else {
           o << "  { const bool ok = (g0 & " << hex64(op.lane_cmask) << ") == " << hex64(op.lane_cmask) << ";\n"

@@ -146,20 +146,22 @@ __device__ __forceinline__ void u1_slot(Regs<R, RB>& a, const Coef4 u, uint32_t 
   }
 }
 
-// X on register slot J in the lanes where `ok` holds (controls held by threads): a per-lane
-// select of the two amplitudes (compiled stages, where J is a constant).
+// X on register slot J in the lanes where `ok` holds (controls held by threads; compiled
+// stages, where J is a constant): a branch on `ok` in which the lanes that take it exchange
+// the amplitudes. Moves under the lane mask (30-qubit brickwork 156.6 -> 150.9 ms) cost less
+// than a select of every 32-bit half in every lane.
 template <typename R, int RB, int J, bool HAS_NS>
-__device__ __forceinline__ void u1_xsel(Regs<R, RB>& a, bool ok, uint32_t ns) {
+__device__ __forceinline__ void u1_xswap(Regs<R, RB>& a, bool ok, uint32_t ns) {
+  if (ok) {
 #pragma unroll
-  for (int m0 = 0; m0 < (1 << RB); ++m0) {
-    if (m0 & (1 << J)) continue;
-    if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
-    const int m1 = m0 | (1 << J);
-    const typename CT<R>::T x = a.v[m0], y = a.v[m1];
-    a.v[m0].x = ok ? y.x : x.x;
-    a.v[m0].y = ok ? y.y : x.y;
-    a.v[m1].x = ok ? x.x : y.x;
-    a.v[m1].y = ok ? x.y : y.y;
+    for (int m0 = 0; m0 < (1 << RB); ++m0) {
+      if (m0 & (1 << J)) continue;
+      if (HAS_NS && ((uint32_t)m0 & ns) != ns) continue;
+      const int m1 = m0 | (1 << J);
+      const typename CT<R>::T x = a.v[m0];
+      a.v[m0].x = a.v[m1].x; a.v[m0].y = a.v[m1].y;
+      a.v[m1].x = x.x; a.v[m1].y = x.y;
+    }
   }
 }
 
